@@ -1,0 +1,199 @@
+"""Checks that pin the oracle where the golden dumps leave room: closed forms,
+invariants and brute force on tiny lattices.  Each would fail for a dropped
+term, a wrong sign, or a transposed index in the function it names."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle, hash_uniform, lib
+
+CX = np.array([1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0, 0])
+CY = np.array([0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0])
+CZ = np.array([0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0])
+G = np.array([2] * 6 + [1] * 13, dtype=float)
+C3 = np.stack([CX, CY, CZ], axis=1).astype(float)
+
+
+def boltz(bdist, u, du=(0.0, 0.0, 0.0)):
+    out = (ctypes.c_double * 19)()
+    lib().or_boltz_dist(bdist, *[float(v) for v in u], *[float(v) for v in du], out)
+    return np.array(out[:])
+
+
+def test_lattice_tensors():
+    W = np.array([1 / 36] * 18 + [1 / 3])
+    assert np.sum(G * W) == pytest.approx(1.0, abs=1e-16)
+    np.testing.assert_allclose(np.einsum("i,i,ia,ib->ab", G, W, C3, C3), np.eye(3) / 3, atol=1e-16)
+    # the Shan-Chen stencil sum_i g_i c_i c_i = 12 I used by the linear-field test
+    np.testing.assert_allclose(np.einsum("i,ia,ib->ab", G[:18], C3[:18], C3[:18]), 12 * np.eye(3))
+
+
+@pytest.mark.parametrize("bdist", [0, 2])
+def test_equilibrium_moments(bdist):
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        u = rng.uniform(-0.08, 0.08, 3)
+        du = rng.uniform(-0.02, 0.02, 3)
+        b = boltz(bdist, u, du)
+        assert np.sum(G * b) == pytest.approx(1.0, abs=1e-15)
+        np.testing.assert_allclose(np.einsum("i,i,ia->a", G, b, C3), u + du, atol=2e-16)
+    b0 = boltz(bdist, (0, 0, 0))
+    np.testing.assert_array_equal(b0, np.array([1 / 36] * 18 + [1 / 3]))
+
+
+def test_bdist2_known_values():
+    """Third-order Taylor form at u=(0.1,0,0): axis pair with s=0.3."""
+    b = boltz(2, (0.1, 0.0, 0.0))
+    uke = 0.5 * 0.1 * 0.3
+    w = 1 / 36
+    even = w * (1 - uke) + w / 2 * 0.09
+    odd = w * (1 - uke) * 0.3 + w / 6 * 0.027
+    assert b[0] == pytest.approx(even + odd, abs=1e-17)
+    assert b[1] == pytest.approx(even - odd, abs=1e-17)
+    assert b[2] == pytest.approx(w * (1 - uke), abs=1e-17)
+    assert b[18] == pytest.approx((1 - uke) / 3, abs=1e-17)
+
+
+def test_streaming_is_the_pull_permutation():
+    """Advection alone on a 4x5x3 lattice with unique values: n_i(x) equals the
+    pre-advection n_i(x - c_i), periodic wrap; the rest population stays."""
+    nx, ny, nz = 4, 5, 3
+    o = Oracle(nx, ny, nz, ncomp=2)
+    N = o.N
+    vals = np.arange(nz * ny * nx * 38, dtype=float).reshape(nz, ny, nx, 38) + 1.0
+    N[1:-1, 1:-1, 1:-1, 0:19] = vals[..., :19]
+    N[1:-1, 1:-1, 1:-1, 24:43] = vals[..., 19:]
+    before = o.populations().copy()
+    o.bc_apply(6)
+    o.advection()
+    after = o.populations()
+    for c in range(2):
+        for i in range(19):
+            exp = np.roll(before[c, i], shift=(CZ[i], CY[i], CX[i]), axis=(0, 1, 2))
+            np.testing.assert_array_equal(after[c, i], exp)
+    # moments written by the advection: rho = sum g n
+    I = o.interior()
+    np.testing.assert_allclose(I[..., 19], np.einsum("i,izyx->zyx", G, after[0]), rtol=1e-15)
+
+
+def test_sc_force_linear_psi_closed_form():
+    """psi_b linear in space => sum_i g_i psi_b(x+c_i) c_i = 12 grad psi_b and
+    F_r = -g_br psi_r(x) 12 grad psi_b (binary, code/lb3d_force_shanchen.F90:162-208)."""
+    o = Oracle(8, 8, 8, ncomp=2, g_br=0.09)
+    N = o.N
+    z, y, x = np.meshgrid(np.arange(10), np.arange(10), np.arange(10), indexing="ij")
+    grad = np.array([0.011, -0.007, 0.004])
+    psi_b = 0.3 + grad[0] * x + grad[1] * y + grad[2] * z
+    psi_r = 0.45 - 0.5 * grad[0] * x + 0.2 * grad[2] * z
+    N[..., 43] = -np.log1p(-psi_b)
+    N[..., 19] = -np.log1p(-psi_r)
+    f = o.sc_force(4, 5, 3)
+    pr = psi_r[3, 5, 4]
+    pb = psi_b[3, 5, 4]
+    np.testing.assert_allclose(f[0], -0.09 * pr * 12 * grad, rtol=1e-11, atol=1e-16)
+    grad_r = np.array([-0.5 * grad[0], 0.0, 0.2 * grad[2]])
+    np.testing.assert_allclose(f[1], -0.09 * pb * 12 * grad_r, rtol=1e-11, atol=1e-16)
+
+
+def test_rest_state_is_a_fixed_point():
+    for ncomp in (2, 3):
+        o = Oracle(6, 5, 4, ncomp=ncomp, bdist=2)
+        o.init_noise(seed=3, rho_mean=(0.7, 0.6, 0.5), amp=(0.0, 0.0, 0.0), dipoles=False)
+        o.bc_init()
+        before = o.populations().copy()
+        o.run(3)
+        np.testing.assert_allclose(o.populations(), before, rtol=1e-14, atol=0)
+
+
+def test_mass_exact_per_component_and_determinism():
+    o = Oracle(12, 10, 8, ncomp=2)
+    o.init_noise(seed=7)
+    o.bc_init()
+    m0, p0, _ = o.sanity()
+    o.run(20)
+    m1, p1, c = o.sanity()
+    np.testing.assert_allclose(m1, m0, rtol=1e-13)
+    # total momentum of a periodic system with pair-antisymmetric forces stays ~0
+    assert np.all(np.abs(p1) < 1e-12)
+    o2 = Oracle(12, 10, 8, ncomp=2)
+    o2.init_noise(seed=7)
+    o2.bc_init()
+    o2.run(20)
+    np.testing.assert_array_equal(o2.populations(), o.populations())
+
+
+def test_le_uniform_rest_fluid_halo_is_shifted_equilibrium():
+    """A uniform fluid at rest: the x=0 halo becomes rho*feq(0,0,-2U) and the
+    x=nx+1 halo rho*feq(0,0,+2U) (code/lb3d_bc_leesedwards.F90:682-825, 863-1003)."""
+    U = 0.02
+    o = Oracle(6, 4, 8, ncomp=2, lees_edwards=1, shear_u=U)
+    o.init_noise(seed=1, rho_mean=(0.7, 0.5, 0.0), amp=(0, 0, 0), dipoles=False)
+    o.bc_init()
+    Ueff = U * 6 / 5
+    N = o.N
+    lo = N[1:-1, 1:-1, 0, :]
+    hi = N[1:-1, 1:-1, -1, :]
+    for c, rho in ((0, 0.7), (1, 0.5)):
+        np.testing.assert_allclose(lo[..., 24 * c:24 * c + 19],
+                                   np.broadcast_to(rho * boltz(2, (0, 0, -2 * Ueff)), lo[..., :19].shape),
+                                   rtol=0, atol=1e-16)
+        np.testing.assert_allclose(hi[..., 24 * c:24 * c + 19],
+                                   np.broadcast_to(rho * boltz(2, (0, 0, 2 * Ueff)), hi[..., :19].shape),
+                                   rtol=0, atol=1e-16)
+
+
+def test_le_interpolation_of_a_z_linear_field():
+    """Populations linear in z: the low halo equals the field shifted by
+    +offset (2*shear_sum) in z, the high halo shifted by -offset, before the
+    velocity correction.  Checked on the halo density with U chosen so the
+    correction vanishes (shear_u=0 gives off=0, so use a direct restore)."""
+    nx, ny, nz = 5, 3, 12
+    o = Oracle(nx, ny, nz, ncomp=2, lees_edwards=1, shear_u=0.0, restore=1, shear_sum=1.35)
+    I = o.interior()
+    zz = np.arange(1, nz + 1, dtype=float)[:, None, None]
+    # rest-equilibrium populations with density linear in z inside a window
+    rho = 0.6 + 0.01 * zz + 0 * I[..., 0]
+    W = np.array([1 / 36] * 18 + [1 / 3])
+    I[..., 0:19] = rho[..., None] * W
+    I[..., 24:43] = 0.5 * W
+    o.bc_init()
+    s, lelo, f, _ = o.le_state()
+    off = 2 * s
+    assert lelo == int(off) and f == pytest.approx(off - int(off))
+    N = o.N
+    lo_rho = np.einsum("i,zyi->zy", G, N[1:-1, 1:-1, 0, 0:19])
+    hi_rho = np.einsum("i,zyi->zy", G, N[1:-1, 1:-1, -1, 0:19])
+    # away from the wrap in z, interpolation is exact on a linear field
+    for z in range(1, nz - 4):
+        assert lo_rho[z - 1, 0] == pytest.approx(0.6 + 0.01 * (z + off), abs=1e-14)
+    for z in range(5, nz + 1):
+        assert hi_rho[z - 1, 0] == pytest.approx(0.6 + 0.01 * (z - off), abs=1e-14)
+
+
+def test_dipole_relaxation_langevin_closed_form():
+    """Uniform ternary fluid with uniform dipoles d0: after advection
+    da = rho_s d0, the colour term vanishes and the surfactant sum cancels
+    (sum_i g_i (da - 2 c_i c_i.da) = 0), so h = da and
+    db = w_d d_0 L(beta|h|) h/|h| + (1-w_d) da/rho_s with L = coth - 1/x."""
+    tau_d, beta, d_0 = 2.0, 3.0, 0.8
+    o = Oracle(5, 4, 6, ncomp=3, tau_d=tau_d, beta=beta, d_0=d_0)
+    o.init_noise(seed=2, rho_mean=(0.7, 0.7, 0.6), amp=(0, 0, 0), dipoles=False)
+    d0 = np.array([0.3, -0.5, 0.2])
+    o.interior()[..., 75:78] = d0
+    o.bc_init()
+    o.run(1)
+    h = 0.6 * d0
+    hm = beta * np.linalg.norm(h)
+    L = 1 / np.tanh(hm) - 1 / hm
+    wd = 1 / tau_d
+    expect = wd * d_0 * L * h * beta / hm + (1 - wd) * h / 0.6
+    db = o.dipole()
+    np.testing.assert_allclose(db[:, 2, 1, 3], expect, rtol=1e-13)
+
+
+def test_hash_uniform_range_and_determinism():
+    v = [hash_uniform(1, s, i) for s in range(3) for i in range(200)]
+    assert min(v) >= 0.0 and max(v) < 1.0
+    assert hash_uniform(1, 0, 5) == hash_uniform(1, 0, 5)
+    assert hash_uniform(1, 0, 5) != hash_uniform(2, 0, 5)
