@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""bench.py -- MLUPS and HBM-roofline fraction of the LB3D D3Q19 timestep on
+one MI355X, FP32 deviation storage against the plain FP64 path.
+
+Workload (the spinodal-decomposition configuration): binary immiscible
+Shan-Chen, D3Q19 BGK, 256^3 periodic, tau = 1, g_br = 0.08, bdist 2, initial
+densities 0.7 + U(-0.01, 0.01) per component and site (seed 1), zero velocity.
+Both forms start from the same device-generated state; the timed region is
+`--steps` whole timesteps (graph replay) with the lattice resident in HBM,
+bracketed by stream synchronisation.  `value` is the FP32-deviation MLUPS;
+the FP64-plain MLUPS, the speedup and the density error between the two are
+reported beside it.  Per-kernel durations for the roofline come from HIP
+events on the library's own stream.  The CPU baseline is the oracle (the FP64
+restatement of the reference loop, one core) timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + % HBM roofline, FP32-batched vs FP64-plain D3Q19, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_site(ncomp, ternary, real_bytes):
+    """Algorithmic bytes per site update: read and write every stored
+    population once, plus the ternary dipole field (3 reals)."""
+    return 2 * (19 * ncomp + (3 if ternary else 0)) * real_bytes
+
+
+def run_form(n, ncomp, precision, steps, warmup, profile_steps):
+    from lb3d_amd.gpu import Lattice
+
+    g = Lattice(n, n, n, ncomp=ncomp, precision=precision, g_br=0.08, bdist=2)
+    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
+    g.bc_init()
+    g.time_loop(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.time_loop(steps)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    rho = g.density()
+    kern, ms_step_prof = g.profile_steps(profile_steps)
+    m, _, counts = g.sanity_check()
+    g.close()
+    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, mlups=n ** 3 * steps / dt / 1e6,
+                kernels=kern, prof_ms_per_step=ms_step_prof, rho=rho, mass=m.tolist(),
+                counts=counts.tolist())
+
+
+def cpu_baseline(sample_n, sample_steps):
+    from oracle.pyoracle import Oracle
+
+    o = Oracle(sample_n, sample_n, sample_n, ncomp=2, g_br=0.08, bdist=2)
+    o.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
+    o.bc_init()
+    t0 = time.perf_counter()
+    o.run(sample_steps)
+    dt = time.perf_counter() - t0
+    return dict(value=sample_n ** 3 * sample_steps / dt / 1e6, unit="MLUPS", cores=1, kind="port",
+                sample=f"oracle (FP64 C restatement of the reference loop, one core), binary "
+                       f"{sample_n}^3 periodic, {sample_steps} steps, same physics as the workload",
+                seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-sample-size", type=int, default=64)
+    ap.add_argument("--cpu-sample-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path is built and measured on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64
+
+    n, ncomp = args.size, 2
+    print(f"[bench] FP64 plain {n}^3 binary: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    r64 = run_form(n, ncomp, LB3D_FP64, args.steps, args.warmup, args.profile_steps)
+    print(f"[bench] FP64 {r64['mlups']:.1f} MLUPS {r64['kernels']}", file=sys.stderr, flush=True)
+    print(f"[bench] FP32 deviation {n}^3 binary", file=sys.stderr, flush=True)
+    r32 = run_form(n, ncomp, LB3D_FP32_DEV, args.steps, args.warmup, args.profile_steps)
+    print(f"[bench] FP32 {r32['mlups']:.1f} MLUPS {r32['kernels']}", file=sys.stderr, flush=True)
+
+    d = np.abs(r32["rho"] - r64["rho"])
+    ref = np.abs(r64["rho"])
+    acc = dict(steps=args.warmup + args.steps,
+               linf_rel=float(d.max() / ref.max()),
+               l2_rel=float(np.sqrt((d ** 2).sum() / (ref ** 2).sum())),
+               mass_rel=float(max(abs(a - b) / abs(b) for a, b in zip(r32["mass"][:ncomp], r64["mass"][:ncomp]))))
+
+    ns = n ** 3
+    B32 = bytes_per_site(ncomp, False, 4)
+    B64 = bytes_per_site(ncomp, False, 8)
+    kc = r32["kernels"]["collide"]
+    achieved = B32 * ns / (kc * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_collide (FP32 deviation)",
+                    algorithmic_bytes_per_launch=B32 * ns, mean_launch_ms=kc,
+                    step_frac=round(r32["mlups"] * 1e6 * B32 / (HBM_PEAK_GBS * 1e9), 4))
+    fp64 = dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
+                step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
+                kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()})
+    cpu = None
+    if not args.no_cpu_baseline:
+        print(f"[bench] CPU baseline oracle {args.cpu_sample_size}^3 x{args.cpu_sample_steps}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_steps)
+        cpu["value"] = round(cpu["value"], 4)
+    line = {
+        "metric": METRIC,
+        "value": round(r32["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"binary Shan-Chen spinodal D3Q19 {n}^3 periodic, g_br=0.08, tau=1, bdist=2, "
+                               "rho=0.7+-0.01 (seed 1), FP32 deviation storage",
+                   "lattice": [n, n, n], "components": ncomp, "bytes_per_site": B32},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_plain": fp64,
+        "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
+        "accuracy_fp32_vs_fp64": acc,
+        "kernels_ms": {k: round(v, 5) for k, v in r32["kernels"].items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

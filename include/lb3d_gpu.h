@@ -1,0 +1,166 @@
+/*
+ * lb3d_gpu.h -- C ABI of the MI355X (gfx950) LB3D timestep library
+ * (lb3d_amd/liblb3d_gpu.so).
+ *
+ * This is the drop-in boundary for the per-timestep lattice update of LB3D.
+ * The reference's interface is a set of argument-less Fortran module
+ * procedures that mutate the global lattice whole_N:
+ *   lb3d_time_loop()          code/lb3d_time_loop.F90:42-97
+ *   lb3d_bc_init(stage)       code/lb3d_bc.F90:48-90
+ *   lb3d_bc_apply(stage)      code/lb3d_bc.F90:95-155
+ *   lb3d_advection()          code/lb3d_advection.F90:40-50
+ *   lb3d_collision()          code/lb3d_collision.F90:41-47
+ *   lb3d_force_apply(stage)   code/lb3d_force.F90:98-121
+ *   lbe_sanity_check(N, ...)  code/lb3d_sanity.F90:79-384
+ * Each entry point below replaces one of them; the Fortran host binds them
+ * with ISO_C_BINDING (lb3d_amd/fortran/lb3d_gpu_module.F90) and keeps the
+ * reference's names and stage order.  The lattice lives in device memory,
+ * owned by the context; the host's array-of-structures whole_N is touched
+ * only at upload/download.
+ *
+ * Conventions: every function returns 0 on success and a nonzero LB3D_E*
+ * code on failure; lb3d_gpu_last_error() then describes the failure.  The
+ * reference aborts through abend() -> MPI_Abort (code/lb3d_mpi.F90:670-676);
+ * the Fortran host does the same on a nonzero code.  All work of a context
+ * runs in order on one HIP stream owned by the context; upload, download and
+ * diagnostics synchronise that stream.  One host thread per context.
+ */
+#ifndef LB3D_GPU_H
+#define LB3D_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LB3D_ABI_VERSION 1
+
+/* return codes */
+#define LB3D_OK 0
+#define LB3D_EINVAL 1   /* invalid argument or parameter combination */
+#define LB3D_EHIP 2     /* a HIP runtime call failed */
+#define LB3D_ESTATE 3   /* call out of the stage order (e.g. collision before advection) */
+#define LB3D_ENOMEM 4   /* device or host allocation failed */
+
+/* lb3d_params.precision */
+#define LB3D_FP64 0     /* plain FP64 populations, the reference's arithmetic */
+#define LB3D_FP32_DEV 1 /* FP32 deviations n_i - w_i*rho0 from the rest equilibrium */
+
+/* lb3d_params.boundary */
+#define LB3D_BC_PERIODIC 0     /* boundary_cond = 0 */
+#define LB3D_BC_LEES_EDWARDS 1 /* boundary_cond != 0 and inv_fluid = 5 */
+
+/* Physics and lattice parameters (bind(C) compatible, no padding holes).
+ * Field meanings follow the reference's namelists (code/lb3d_config.F90:
+ * 244-297, 319-335); omega = 1/tau is derived (code/lb3d_config.F90:886-889). */
+typedef struct lb3d_params {
+    int32_t nx, ny, nz;       /* global lattice (tnx, tny, tnz); one GPU holds it all */
+    int32_t ncomp;            /* 2 = binary (-DNOSURFACTANT), 3 = ternary (default build) */
+    int32_t bdist;            /* equilibrium: 0 (Luo) or 2 (third-order Taylor) */
+    int32_t precision;        /* LB3D_FP64 or LB3D_FP32_DEV */
+    int32_t boundary;         /* LB3D_BC_PERIODIC or LB3D_BC_LEES_EDWARDS (binary only) */
+    int32_t restore;          /* 1: shear_sum was read from a checkpoint (code/lb3d_bc.F90:67-72) */
+    double amass[3];          /* amass_r, amass_b, amass_s */
+    double tau[3];            /* tau_r, tau_b, tau_s */
+    double tau_d, beta, d_0;  /* dipole relaxation time, inverse temperature, dipole modulus */
+    double g_br, g_bs, g_ss;  /* Shan-Chen couplings */
+    double shear_u;           /* LE wall speed as in the input file (x tnx/(tnx-1) applied here) */
+    double shear_omega;       /* LE oscillation frequency */
+    double shear_sum;         /* LE offset accumulator: checktopo value on restore, else 0 */
+    double g_accn[3];         /* body force acceleration (g_accn_x, g_accn_y, g_accn) */
+    int32_t g_accn_min[3];    /* body-force box, global 1-based inclusive (x, y, z); */
+    int32_t g_accn_max[3];    /*   max 0 means the lattice extent (code/lb3d_config.F90:931-933) */
+    double rho0[3];           /* FP32_DEV reference densities (0 picks 0.7) */
+} lb3d_params;
+
+typedef struct lb3d_ctx lb3d_ctx;
+
+/* Version of this ABI; the Fortran and Python bindings check it. */
+int lb3d_gpu_abi_version(void);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *lb3d_gpu_last_error(void);
+
+/* Create a context on the current HIP device: validates the parameters and
+ * allocates the device lattice.  Replaces the allocation of whole_N at init
+ * stage 3 (code/lb3d_lattice.F90:244-309). */
+int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out);
+int lb3d_gpu_destroy(lb3d_ctx *ctx);
+
+/* Copy the host lattice to the device.  whole_N is the reference's AoS array
+ * of lbe_site records (code/lb3d_lattice.F90:35-83), Fortran order with x
+ * fastest, extent [1-halo, n+halo] in each dimension; site_doubles is 49
+ * (binary) or 79 (ternary).  Populations and (ternary) db of the interior are
+ * read; the halo, rho, u, da and rock_state are ignored (rock must be 0).
+ * Sets the step counter nt = nt0 (tstart). */
+int lb3d_gpu_upload_sites(lb3d_ctx *ctx, const double *whole_N, int32_t halo_extent,
+                          int32_t site_doubles, int32_t nt0);
+/* Copy the lattice back into whole_N: interior populations, the moments of
+ * the last advection (rho; u is the momentum of the current populations),
+ * da and db; the halo is filled with periodic images of the interior (the
+ * layout the reference's dumps see, code/lb3d_mpi.F90:788-858). */
+int lb3d_gpu_download_sites(lb3d_ctx *ctx, double *whole_N, int32_t halo_extent,
+                            int32_t site_doubles);
+
+/* Seeded synthetic state on the device (not a reference routine): rest
+ * equilibrium n_i = w_i*rho with rho = rho_mean[c] + amp[c]*(2u-1), u a
+ * counter-based hash of (seed, c, site); ternary dipoles uniform on the unit
+ * sphere when dipoles != 0.  Sets nt = 0. */
+int lb3d_gpu_init_noise(lb3d_ctx *ctx, uint64_t seed, const double rho_mean[3],
+                        const double amp[3], int32_t dipoles);
+
+/* Stage-4 boundary initialisation (code/lb3d_bc.F90:48-90): for LE rescales
+ * shear_u by tnx/(tnx-1), applies the restore correction and the first
+ * le_neighbours.  Call once after upload/init_noise. */
+int lb3d_gpu_bc_init(lb3d_ctx *ctx);
+
+/* The reference's per-step call surface, in the order of
+ * code/lb3d_time_loop.F90:58-92:
+ *   force_apply(6) bc_apply(6, nt) advection() bc_apply(8, nt) force_apply(8) collision()
+ * Out-of-order calls return LB3D_ESTATE.  Periodic boundaries are index wraps
+ * inside the kernels, so bc_apply only does work for Lees-Edwards. */
+int lb3d_gpu_force_apply(lb3d_ctx *ctx, int32_t stage);          /* code/lb3d_force.F90:98-121 (no-op) */
+int lb3d_gpu_bc_apply(lb3d_ctx *ctx, int32_t stage, int32_t nt); /* code/lb3d_bc.F90:95-155 */
+int lb3d_gpu_advection(lb3d_ctx *ctx);                           /* code/lb3d_advection.F90:40-151 */
+int lb3d_gpu_collision(lb3d_ctx *ctx);                           /* code/lb3d_collision.F90:41-291 */
+
+/* nsteps whole timesteps nt+1 .. nt+nsteps (the body of lb3d_time_loop,
+ * code/lb3d_time_loop.F90:58-92), replayed from a captured HIP graph. */
+int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
+
+/* Current step counter nt. */
+int lb3d_gpu_get_nt(lb3d_ctx *ctx, int32_t *nt);
+
+/* Sanity reductions in FP64 (code/lb3d_sanity.F90:79-224): mass[c] = sum of
+ * rho_c, momentum = total mass flow, counts = {sites with a NaN population,
+ * sites with a negative population, sites with |u_c| > 0.1}.  Synchronises. */
+int lb3d_gpu_diagnostics(lb3d_ctx *ctx, double mass[3], double momentum[3], int64_t counts[3]);
+
+/* Lees-Edwards scalars after the last le_neighbours (code/lb3d_bc_leesedwards.F90:86-140). */
+int lb3d_gpu_le_state(lb3d_ctx *ctx, double *shear_sum, int32_t *lelo, double *lefrac,
+                      double *shear_u);
+
+/* Populations of the interior as the solver holds them, in FP64:
+ * out[((c*19 + i)*nz + z)*ny*nx + y*nx + x] (FP32_DEV: w_i*rho0 + deviation). */
+int lb3d_gpu_get_populations(lb3d_ctx *ctx, double *out);
+
+/* Densities rho_c of the interior in FP64, out[(c*nz + z)*ny*nx + y*nx + x]
+ * (the od/wd/sur fields of code/lb3d_io_dump_data.F90:922-1166 without
+ * amass).  Synchronises. */
+int lb3d_gpu_get_density(lb3d_ctx *ctx, double *out);
+
+/* Wait for all work queued on the context's stream. */
+int lb3d_gpu_synchronize(lb3d_ctx *ctx);
+
+/* Timing support: runs nsteps timesteps with HIP events around every kernel
+ * on the context's stream and returns the mean duration per launch.
+ * names/ms have room for max_kernels entries; *nkernels is set to the count. */
+int lb3d_gpu_profile_steps(lb3d_ctx *ctx, int32_t nsteps, int32_t max_kernels,
+                           const char **names, double *ms, int32_t *nkernels,
+                           double *ms_per_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,681 @@
+// lb3d_gpu.hip -- the C ABI of include/lb3d_gpu.h: contexts, device memory,
+// the reference's stage sequence, HIP-graph replay, diagnostics and the
+// conversion between the reference's AoS lattice and the device layout.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lb3d_kernels.h"
+#include "lb3d_solver.h"
+
+using namespace lb3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(LB3D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+enum StageState { IDLE = 0, AFTER_BC6 = 1, AFTER_ADV = 2, AFTER_BC8 = 3 };
+
+const char *const KNAMES[] = {"le_neighbours", "le_faces_pc", "moments",
+                              "le_faces_aux",  "dipole",      "collide"};
+enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_COUNT };
+
+}  // namespace
+
+struct lb3d_ctx {
+    lb3d_params p{};
+    Launcher *L = nullptr;
+    DevBufs b;
+    hipStream_t stream = nullptr;
+    int nt = 0;
+    int state = IDLE;
+    bool le = false;
+    bool aux_valid = false;
+    LEState host_le{};
+    hipGraphExec_t graph = nullptr;
+    double *d_partials = nullptr;
+    size_t f_bytes = 0, aux_bytes = 0, db_bytes = 0;
+    int naux = 0;
+};
+
+static int check_params(const lb3d_params *p)
+{
+    if (!p) return fail(LB3D_EINVAL, "params is NULL");
+    if (p->nx < 2 || p->ny < 1 || p->nz < 1)
+        return fail(LB3D_EINVAL, "lattice extent must be nx>=2, ny>=1, nz>=1");
+    if (p->ncomp != 2 && p->ncomp != 3) return fail(LB3D_EINVAL, "ncomp must be 2 or 3");
+    if (p->bdist != 0 && p->bdist != 2)
+        return fail(LB3D_EINVAL, "bdist must be 0 (Luo) or 2 (third-order Taylor)");
+    if (p->precision != LB3D_FP64 && p->precision != LB3D_FP32_DEV)
+        return fail(LB3D_EINVAL, "precision must be LB3D_FP64 or LB3D_FP32_DEV");
+    if (p->boundary != LB3D_BC_PERIODIC && p->boundary != LB3D_BC_LEES_EDWARDS)
+        return fail(LB3D_EINVAL, "boundary must be periodic or Lees-Edwards");
+    if (p->boundary == LB3D_BC_LEES_EDWARDS && p->ncomp == 3)
+        // code/lb3d_compatibility.F90:105-108
+        return fail(LB3D_EINVAL, "Lees-Edwards boundaries are not supported with surfactant");
+    for (int c = 0; c < p->ncomp; c++)
+        if (!(p->tau[c] > 0.0) || !(p->amass[c] > 0.0))
+            return fail(LB3D_EINVAL, "tau and amass must be positive");
+    if (p->ncomp == 3 && !(p->tau_d > 0.0)) return fail(LB3D_EINVAL, "tau_d must be positive");
+    double sites = (double)p->nx * (double)p->ny * (double)p->nz;
+    if (sites * 19.0 * p->ncomp >= 4294967296.0)
+        return fail(LB3D_EINVAL, "lattice too large for 32-bit population indexing");
+    return LB3D_OK;
+}
+
+extern "C" {
+
+int lb3d_gpu_abi_version(void) { return LB3D_ABI_VERSION; }
+
+const char *lb3d_gpu_last_error(void) { return g_err.c_str(); }
+
+int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
+{
+    g_err.clear();
+    if (!out) return fail(LB3D_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    lb3d_ctx *c = new lb3d_ctx();
+    c->p = *p;
+    c->le = p->boundary == LB3D_BC_LEES_EDWARDS;
+    c->L = p->precision == LB3D_FP64 ? make_launcher_fp64(*p) : make_launcher_fp32(*p);
+    size_t rs = c->L->real_bytes();
+    size_t ns = (size_t)p->nx * p->ny * p->nz;
+    c->naux = p->ncomp == 3 ? NAUX3 : NAUX2;
+    c->f_bytes = (size_t)p->ncomp * 19 * ns * rs;
+    c->aux_bytes = (size_t)c->naux * ns * rs;
+    c->db_bytes = p->ncomp == 3 ? 3 * ns * rs : 0;
+    size_t plane = (size_t)p->ny * p->nz;
+    auto cleanup = [&](int code, const std::string &m) {
+        lb3d_gpu_destroy(c);
+        return fail(code, m);
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return cleanup(LB3D_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    for (int k = 0; k < 2; k++)
+        if ((e = hipMalloc(&c->b.f[k], c->f_bytes)) != hipSuccess)
+            return cleanup(LB3D_ENOMEM, std::string("hipMalloc populations: ") + hipGetErrorString(e));
+    if ((e = hipMalloc(&c->b.aux, c->aux_bytes)) != hipSuccess)
+        return cleanup(LB3D_ENOMEM, std::string("hipMalloc aux: ") + hipGetErrorString(e));
+    if (c->db_bytes && (e = hipMalloc(&c->b.db, c->db_bytes)) != hipSuccess)
+        return cleanup(LB3D_ENOMEM, std::string("hipMalloc db: ") + hipGetErrorString(e));
+    if (c->le) {
+        if ((e = hipMalloc(&c->b.face_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
+            (e = hipMalloc(&c->b.face_aux, 2 * 4 * plane * rs)) != hipSuccess)
+            return cleanup(LB3D_ENOMEM, std::string("hipMalloc LE planes: ") + hipGetErrorString(e));
+    }
+    if ((e = hipMalloc((void **)&c->b.le, sizeof(LEState))) != hipSuccess ||
+        (e = hipMalloc((void **)&c->d_partials, (size_t)c->L->diag_blocks() * NDIAG * sizeof(double))) !=
+            hipSuccess)
+        return cleanup(LB3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->host_le.shear_u = p->shear_u;
+    c->host_le.shear_tmp = p->shear_u;
+    c->host_le.shear_sum = p->shear_sum;
+    c->host_le.shear_omega = p->shear_omega;
+    *out = c;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_destroy(lb3d_ctx *c)
+{
+    if (!c) return LB3D_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (int k = 0; k < 2; k++)
+        if (c->b.f[k]) (void)hipFree(c->b.f[k]);
+    if (c->b.aux) (void)hipFree(c->b.aux);
+    if (c->b.db) (void)hipFree(c->b.db);
+    if (c->b.face_pc) (void)hipFree(c->b.face_pc);
+    if (c->b.face_aux) (void)hipFree(c->b.face_aux);
+    if (c->b.le) (void)hipFree(c->b.le);
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c->L;
+    delete c;
+    return LB3D_OK;
+}
+
+static int push_le(lb3d_ctx *c)
+{
+    HIP_TRY(hipMemcpyAsync(c->b.le, &c->host_le, sizeof(LEState), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LB3D_OK;
+}
+
+static int pull_le(lb3d_ctx *c)
+{
+    HIP_TRY(hipMemcpyAsync(&c->host_le, c->b.le, sizeof(LEState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LB3D_OK;
+}
+
+static bool is_dev(const lb3d_ctx *c) { return c->p.precision == LB3D_FP32_DEV; }
+static double rho0_of(const lb3d_ctx *c, int k) { return c->p.rho0[k] > 0.0 ? c->p.rho0[k] : 0.7; }
+static double w_of(int i) { return i < 18 ? 1.0 / 36.0 : 1.0 / 3.0; }
+
+int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, int32_t nt0)
+{
+    g_err.clear();
+    if (!c || !N) return fail(LB3D_EINVAL, "NULL argument");
+    const int C = c->p.ncomp;
+    if (sd != (C == 3 ? 79 : 49)) return fail(LB3D_EINVAL, "site_doubles must be 79 (ternary) or 49 (binary)");
+    if (h < 1) return fail(LB3D_EINVAL, "halo_extent must be >= 1");
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const size_t ns = (size_t)nx * ny * nz;
+    const size_t lx = nx + 2 * h, ly = ny + 2 * h;
+    const int rock_off = C == 3 ? 78 : 48;
+    const bool dev = is_dev(c);
+    std::vector<double> fd(dev ? 0 : (size_t)C * 19 * ns);
+    std::vector<float> ff(dev ? (size_t)C * 19 * ns : 0);
+    std::vector<double> dbd(!dev && C == 3 ? 3 * ns : 0);
+    std::vector<float> dbf(dev && C == 3 ? 3 * ns : 0);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const double *S = N + ((size_t)(x + h) + lx * ((size_t)(y + h) + ly * (size_t)(z + h))) * sd;
+                size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
+                if (S[rock_off] != 0.0) return fail(LB3D_EINVAL, "rock sites are not supported (rock_state must be 0)");
+                for (int k = 0; k < C; k++)
+                    for (int i = 0; i < 19; i++) {
+                        double v = S[24 * k + i];
+                        size_t q = ((size_t)k * 19 + i) * ns + s;
+                        if (dev)
+                            ff[q] = (float)(v - w_of(i) * rho0_of(c, k));
+                        else
+                            fd[q] = v;
+                    }
+                if (C == 3)
+                    for (int k = 0; k < 3; k++) {
+                        if (dev)
+                            dbf[k * ns + s] = (float)S[75 + k];
+                        else
+                            dbd[k * ns + s] = S[75 + k];
+                    }
+            }
+    c->b.cur = 0;
+    HIP_TRY(hipMemcpyAsync(c->b.f[0], dev ? (const void *)ff.data() : (const void *)fd.data(), c->f_bytes,
+                           hipMemcpyHostToDevice, c->stream));
+    if (C == 3)
+        HIP_TRY(hipMemcpyAsync(c->b.db, dev ? (const void *)dbf.data() : (const void *)dbd.data(), c->db_bytes,
+                               hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->nt = nt0;
+    c->host_le.nt = nt0;
+    c->state = IDLE;
+    c->aux_valid = false;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_init_noise(lb3d_ctx *c, uint64_t seed, const double rho_mean[3], const double amp[3],
+                        int32_t dipoles)
+{
+    g_err.clear();
+    if (!c || !rho_mean || !amp) return fail(LB3D_EINVAL, "NULL argument");
+    c->b.cur = 0;
+    c->L->init_noise(c->b, seed, rho_mean, amp, dipoles, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->nt = 0;
+    c->host_le.nt = 0;
+    c->state = IDLE;
+    c->aux_valid = false;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_get_populations(lb3d_ctx *c, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    const int C = c->p.ncomp;
+    const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    if (is_dev(c)) {
+        std::vector<float> h((size_t)C * 19 * ns);
+        HIP_TRY(hipMemcpyAsync(h.data(), c->b.f[c->b.cur], c->f_bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < C; k++)
+            for (int i = 0; i < 19; i++) {
+                double base = w_of(i) * rho0_of(c, k);
+                size_t off = ((size_t)k * 19 + i) * ns;
+                for (size_t s = 0; s < ns; s++) out[off + s] = base + (double)h[off + s];
+            }
+    } else {
+        HIP_TRY(hipMemcpyAsync(out, c->b.f[c->b.cur], c->f_bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
+{
+    g_err.clear();
+    if (!c || !N) return fail(LB3D_EINVAL, "NULL argument");
+    const int C = c->p.ncomp;
+    if (sd != (C == 3 ? 79 : 49)) return fail(LB3D_EINVAL, "site_doubles must be 79 (ternary) or 49 (binary)");
+    if (h < 1) return fail(LB3D_EINVAL, "halo_extent must be >= 1");
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const size_t ns = (size_t)nx * ny * nz;
+    std::vector<double> pops((size_t)C * 19 * ns);
+    int rc = lb3d_gpu_get_populations(c, pops.data());
+    if (rc) return rc;
+    const bool dev = is_dev(c);
+    const size_t rs = c->L->real_bytes();
+    std::vector<double> aux, db;
+    auto fetch = [&](const void *src, size_t count, std::vector<double> &dst) -> int {
+        dst.resize(count);
+        if (rs == 8) {
+            HIP_TRY(hipMemcpyAsync(dst.data(), src, count * 8, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        } else {
+            std::vector<float> t(count);
+            HIP_TRY(hipMemcpyAsync(t.data(), src, count * 4, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            for (size_t q = 0; q < count; q++) dst[q] = (double)t[q];
+        }
+        return LB3D_OK;
+    };
+    if (c->aux_valid && (rc = fetch(c->b.aux, (size_t)c->naux * ns, aux))) return rc;
+    if (C == 3 && (rc = fetch(c->b.db, 3 * ns, db))) return rc;
+    const size_t lx = nx + 2 * h, ly = ny + 2 * h, lz = nz + 2 * h;
+    const int rock_off = C == 3 ? 78 : 48;
+    auto rec = [&](size_t x, size_t y, size_t z) { return N + (x + lx * (y + ly * z)) * sd; };
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                double *S = rec(x + h, y + h, z + h);
+                size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
+                std::memset(S, 0, sizeof(double) * sd);
+                for (int k = 0; k < C; k++) {
+                    double rho = 0.0, m[3] = {0.0, 0.0, 0.0};
+                    for (int i = 0; i < 19; i++) {
+                        double v = pops[((size_t)k * 19 + i) * ns + s];
+                        S[24 * k + i] = v;
+                        double ng = v * (double)host::GD[i];
+                        rho = rho + ng;
+                        m[0] = m[0] + ng * (double)host::CX[i];
+                        m[1] = m[1] + ng * (double)host::CY[i];
+                        m[2] = m[2] + ng * (double)host::CZ[i];
+                    }
+                    if (c->aux_valid) {
+                        double r = aux[(size_t)(C + k) * ns + s];
+                        rho = dev ? rho0_of(c, k) + r : r;
+                    }
+                    S[24 * k + 19] = rho;
+                    for (int a = 0; a < 3; a++) S[24 * k + 20 + a] = m[a];
+                }
+                if (C == 3)
+                    for (int a = 0; a < 3; a++) {
+                        S[72 + a] = c->aux_valid ? aux[(size_t)(6 + a) * ns + s] : 0.0;
+                        S[75 + a] = db[(size_t)a * ns + s];
+                    }
+                S[rock_off] = 0.0;
+            }
+    // halo: periodic images of the interior (x, then y over full x, then z)
+    for (size_t z = h; z < (size_t)nz + h; z++)
+        for (size_t y = h; y < (size_t)ny + h; y++)
+            for (int k = 1; k <= h; k++) {
+                std::memcpy(rec(h - k, y, z), rec(nx + h - k, y, z), sizeof(double) * sd);
+                std::memcpy(rec(nx + h - 1 + k, y, z), rec(h - 1 + k, y, z), sizeof(double) * sd);
+            }
+    for (size_t z = h; z < (size_t)nz + h; z++)
+        for (int k = 1; k <= h; k++)
+            for (size_t x = 0; x < lx; x++) {
+                std::memcpy(rec(x, h - k, z), rec(x, ny + h - k, z), sizeof(double) * sd);
+                std::memcpy(rec(x, ny + h - 1 + k, z), rec(x, h - 1 + k, z), sizeof(double) * sd);
+            }
+    for (int k = 1; k <= h; k++)
+        for (size_t y = 0; y < ly; y++)
+            for (size_t x = 0; x < lx; x++) {
+                std::memcpy(rec(x, y, h - k), rec(x, y, nz + h - k), sizeof(double) * sd);
+                std::memcpy(rec(x, y, nz + h - 1 + k), rec(x, y, h - 1 + k), sizeof(double) * sd);
+            }
+    (void)lz;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_bc_init(lb3d_ctx *c)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (c->le) {
+        // code/lb3d_bc.F90:58-78 then le_neighbours (code/lb3d_bc_leesedwards.F90:86-140)
+        LEState &L = c->host_le;
+        L.shear_u = ((double)c->p.nx / ((double)c->p.nx - 1.0)) * c->p.shear_u;
+        L.shear_tmp = L.shear_u;
+        L.shear_sum = c->p.shear_sum;
+        L.shear_omega = c->p.shear_omega;
+        if (c->p.restore) L.shear_sum = L.shear_sum - L.shear_u;
+        L.shear_sum = L.shear_sum + L.shear_u;
+        long t = (long)(2.0 * L.shear_sum);
+        L.lelo = (int)(t % (long)c->p.nz);
+        double lf = 2.0 * L.shear_sum;
+        L.lefrac = std::fabs(lf - std::trunc(lf));
+        L.nt = c->nt;
+        return push_le(c);
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_force_apply(lb3d_ctx *c, int32_t stage)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    // lbe_force_apply: the constant and Kolmogorov forces are outside this path
+    if (stage != 6 && stage != 8 && stage != 10) return fail(LB3D_EINVAL, "force_apply stage must be 6, 8 or 10");
+    return LB3D_OK;
+}
+
+static void launch_le_neighbours(lb3d_ctx *c, int nt_set)
+{
+    k_le_neighbours<<<1, 64, 0, c->stream>>>(c->b.le, c->p.nz, nt_set);
+}
+
+int lb3d_gpu_bc_apply(lb3d_ctx *c, int32_t stage, int32_t nt)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (stage == 6) {
+        if (c->state != IDLE) return fail(LB3D_ESTATE, "bc_apply(6) must follow a collision or init");
+        if (c->le) {
+            launch_le_neighbours(c, nt);
+            c->L->le_faces(c->b, false, c->stream);
+            HIP_TRY(hipGetLastError());
+        }
+        c->nt = nt;
+        c->state = AFTER_BC6;
+    } else if (stage == 8) {
+        if (c->state != AFTER_ADV) return fail(LB3D_ESTATE, "bc_apply(8) must follow advection");
+        if (c->le) {
+            c->L->le_faces(c->b, true, c->stream);
+            HIP_TRY(hipGetLastError());
+        }
+        c->state = AFTER_BC8;
+    } else if (stage == 10) {
+        if (c->state != IDLE) return fail(LB3D_ESTATE, "bc_apply(10) must follow a collision");
+    } else {
+        return fail(LB3D_EINVAL, "bc_apply stage must be 6, 8 or 10");
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_advection(lb3d_ctx *c)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (c->state != AFTER_BC6) return fail(LB3D_ESTATE, "advection must follow bc_apply(6)");
+    c->L->moments(c->b, c->stream);
+    HIP_TRY(hipGetLastError());
+    c->state = AFTER_ADV;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_collision(lb3d_ctx *c)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (c->state != AFTER_BC8) return fail(LB3D_ESTATE, "collision must follow bc_apply(8)");
+    c->L->dipole(c->b, c->stream);
+    c->L->collide(c->b, c->stream);
+    HIP_TRY(hipGetLastError());
+    c->b.cur = 1 - c->b.cur;
+    c->state = IDLE;
+    c->aux_valid = true;
+    return LB3D_OK;
+}
+
+// One whole timestep on the stream (graph-capturable: kernels only).
+static void enqueue_step(lb3d_ctx *c)
+{
+    if (c->le) {
+        launch_le_neighbours(c, -1);
+        c->L->le_faces(c->b, false, c->stream);
+    }
+    c->L->moments(c->b, c->stream);
+    if (c->le) c->L->le_faces(c->b, true, c->stream);
+    c->L->dipole(c->b, c->stream);
+    c->L->collide(c->b, c->stream);
+    c->b.cur = 1 - c->b.cur;
+}
+
+static int ensure_graph(lb3d_ctx *c)
+{
+    if (c->graph) return LB3D_OK;
+    // two steps starting from cur = 0 (A -> B -> A)
+    int saved = c->b.cur;
+    c->b.cur = 0;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c);
+    enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->b.cur = saved;
+    if (e != hipSuccess) return fail(LB3D_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->graph = nullptr;
+        return fail(LB3D_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (nsteps < 0) return fail(LB3D_EINVAL, "nsteps must be >= 0");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "run_steps needs a completed timestep");
+    if (nsteps == 0) return LB3D_OK;
+    if (c->le) {
+        // the device counter follows the host's nt
+        c->host_le.nt = c->nt;
+        int rc = pull_le(c);
+        if (rc) return rc;
+        if (c->host_le.nt != c->nt) {
+            c->host_le.nt = c->nt;
+            if ((rc = push_le(c))) return rc;
+        }
+    }
+    int rc = ensure_graph(c);
+    if (rc) return rc;
+    int left = nsteps;
+    if (c->b.cur == 1) {
+        enqueue_step(c);
+        left--;
+    }
+    for (; left >= 2; left -= 2) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    if (left == 1) enqueue_step(c);
+    HIP_TRY(hipGetLastError());
+    c->nt += nsteps;
+    c->aux_valid = true;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_synchronize(lb3d_ctx *c)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LB3D_OK;
+}
+
+int lb3d_gpu_get_density(lb3d_ctx *c, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    const int C = c->p.ncomp;
+    const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    // one component plane at a time keeps the host buffer small
+    std::vector<double> plane(ns);
+    std::vector<float> planef(is_dev(c) ? ns : 0);
+    const size_t rs = c->L->real_bytes();
+    for (int k = 0; k < C; k++) {
+        double *rho = out + (size_t)k * ns;
+        for (size_t s = 0; s < ns; s++) rho[s] = 0.0;
+        for (int i = 0; i < 19; i++) {
+            const char *src = (const char *)c->b.f[c->b.cur] + ((size_t)k * 19 + i) * ns * rs;
+            if (is_dev(c)) {
+                HIP_TRY(hipMemcpyAsync(planef.data(), src, ns * 4, hipMemcpyDeviceToHost, c->stream));
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                double base = w_of(i) * rho0_of(c, k);
+                for (size_t s = 0; s < ns; s++) rho[s] = rho[s] + (base + (double)planef[s]) * (double)host::GD[i];
+            } else {
+                HIP_TRY(hipMemcpyAsync(plane.data(), src, ns * 8, hipMemcpyDeviceToHost, c->stream));
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                for (size_t s = 0; s < ns; s++) rho[s] = rho[s] + plane[s] * (double)host::GD[i];
+            }
+        }
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_get_nt(lb3d_ctx *c, int32_t *nt)
+{
+    if (!c || !nt) return fail(LB3D_EINVAL, "NULL argument");
+    *nt = c->nt;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_diagnostics(lb3d_ctx *c, double mass[3], double momentum[3], int64_t counts[3])
+{
+    g_err.clear();
+    if (!c || !mass || !momentum || !counts) return fail(LB3D_EINVAL, "NULL argument");
+    unsigned nb = c->L->diag_blocks();
+    c->L->diagnostics(c->b, c->d_partials, c->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nb * NDIAG);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->d_partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double acc[NDIAG] = {0};
+    for (unsigned b = 0; b < nb; b++)
+        for (int k = 0; k < NDIAG; k++) acc[k] = acc[k] + h[(size_t)b * NDIAG + k];
+    for (int k = 0; k < 3; k++) {
+        mass[k] = k < c->p.ncomp ? acc[k] : 0.0;
+        momentum[k] = acc[3 + k];
+        counts[k] = (int64_t)llround(acc[6 + k]);
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_le_state(lb3d_ctx *c, double *shear_sum, int32_t *lelo, double *lefrac, double *shear_u)
+{
+    g_err.clear();
+    if (!c || !shear_sum || !lelo || !lefrac || !shear_u) return fail(LB3D_EINVAL, "NULL argument");
+    if (c->le) {
+        int rc = pull_le(c);
+        if (rc) return rc;
+    }
+    *shear_sum = c->host_le.shear_sum;
+    *lelo = c->host_le.lelo;
+    *lefrac = c->host_le.lefrac;
+    *shear_u = c->host_le.shear_u;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, const char **names, double *ms,
+                           int32_t *nkernels, double *ms_per_step)
+{
+    g_err.clear();
+    if (!c || !names || !ms || !nkernels || !ms_per_step) return fail(LB3D_EINVAL, "NULL argument");
+    if (nsteps < 1 || max_kernels < K_COUNT) return fail(LB3D_EINVAL, "nsteps >= 1 and room for 6 kernels needed");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "profile_steps needs a completed timestep");
+    if (c->le) {
+        c->host_le.nt = c->nt;
+        int rc = pull_le(c);
+        if (rc) return rc;
+        c->host_le.nt = c->nt;
+        if ((rc = push_le(c))) return rc;
+    }
+    std::vector<hipEvent_t> ev((size_t)nsteps * K_COUNT * 2, nullptr);
+    std::vector<int> used((size_t)nsteps * K_COUNT, 0);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    auto rec = [&](int step, int k, bool start) {
+        return hipEventRecord(ev[((size_t)step * K_COUNT + k) * 2 + (start ? 0 : 1)], c->stream);
+    };
+    HIP_TRY(hipEventRecord(t0, c->stream));
+    for (int st = 0; st < nsteps; st++) {
+        if (c->le) {
+            HIP_TRY(rec(st, K_LEN, true));
+            launch_le_neighbours(c, -1);
+            HIP_TRY(rec(st, K_LEN, false));
+            HIP_TRY(rec(st, K_LE6, true));
+            c->L->le_faces(c->b, false, c->stream);
+            HIP_TRY(rec(st, K_LE6, false));
+            used[(size_t)st * K_COUNT + K_LEN] = used[(size_t)st * K_COUNT + K_LE6] = 1;
+        }
+        HIP_TRY(rec(st, K_MOM, true));
+        c->L->moments(c->b, c->stream);
+        HIP_TRY(rec(st, K_MOM, false));
+        used[(size_t)st * K_COUNT + K_MOM] = 1;
+        if (c->le) {
+            HIP_TRY(rec(st, K_LE8, true));
+            c->L->le_faces(c->b, true, c->stream);
+            HIP_TRY(rec(st, K_LE8, false));
+            used[(size_t)st * K_COUNT + K_LE8] = 1;
+        }
+        if (c->p.ncomp == 3) {
+            HIP_TRY(rec(st, K_DIP, true));
+            c->L->dipole(c->b, c->stream);
+            HIP_TRY(rec(st, K_DIP, false));
+            used[(size_t)st * K_COUNT + K_DIP] = 1;
+        }
+        HIP_TRY(rec(st, K_COL, true));
+        c->L->collide(c->b, c->stream);
+        HIP_TRY(rec(st, K_COL, false));
+        used[(size_t)st * K_COUNT + K_COL] = 1;
+        c->b.cur = 1 - c->b.cur;
+    }
+    HIP_TRY(hipEventRecord(t1, c->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double tot[K_COUNT] = {0};
+    int cnt[K_COUNT] = {0};
+    for (int st = 0; st < nsteps; st++)
+        for (int k = 0; k < K_COUNT; k++)
+            if (used[(size_t)st * K_COUNT + k]) {
+                float m = 0.f;
+                HIP_TRY(hipEventElapsedTime(&m, ev[((size_t)st * K_COUNT + k) * 2], ev[((size_t)st * K_COUNT + k) * 2 + 1]));
+                tot[k] += m;
+                cnt[k]++;
+            }
+    float all = 0.f;
+    HIP_TRY(hipEventElapsedTime(&all, t0, t1));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    int nk = 0;
+    for (int k = 0; k < K_COUNT; k++)
+        if (cnt[k]) {
+            names[nk] = KNAMES[k];
+            ms[nk] = tot[k] / cnt[k];
+            nk++;
+        }
+    *nkernels = nk;
+    *ms_per_step = all / nsteps;
+    c->nt += nsteps;
+    c->aux_valid = true;
+    return LB3D_OK;
+}
+
+}  // extern "C"

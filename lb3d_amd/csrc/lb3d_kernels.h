@@ -1,0 +1,587 @@
+// lb3d_kernels.h -- the per-timestep kernels, templated on the storage form.
+//
+// Storage (structure of arrays, x fastest, no halo; periodic boundaries are
+// index wraps):
+//   f[(c*19 + i)*ns + s]   populations, two buffers (pull: read one, write other)
+//   aux[k*ns + s]          moments of the streamed state, written by k_moments:
+//                          binary  k = psi_r, psi_b, rho_r, rho_b
+//                          ternary k = psi_r, psi_b, psi_s, rho_r, rho_b, rho_s, da_x, da_y, da_z
+//   db[k*ns + s]           ternary dipole field
+//   face_pc[((f*C + c)*19 + i)*ny*nz + z*ny + y]  Lees-Edwards x-halo planes
+//                          (f = 0: x=0 plane, f = 1: x=nx+1 plane) of the
+//                          post-collision populations, built by k_le_faces<false>
+//   face_aux[(f*4 + k)*ny*nz + z*ny + y]  moments of the post-advection
+//                          x-halo planes (binary), built by k_le_faces<true>
+//
+// Form FP64 (Dev = false, R = double): the reference's values and arithmetic
+// order; compile with -ffp-contract=off.  Form FP32_DEV (Dev = true,
+// R = float): f holds n_i - w_i*rho0_c, rho entries hold rho - rho0_c, psi
+// entries hold psi(rho) - psi(rho0_c).
+#pragma once
+
+#include "lb3d_lattice.h"
+
+namespace lb3d {
+
+constexpr int NAUX2 = 4;
+constexpr int NAUX3 = 9;
+
+__device__ __forceinline__ void site_xyz(uint32_t s, int nx, int ny, int &x, int &y, int &z)
+{
+    uint32_t t = s / (uint32_t)nx;
+    x = (int)(s - t * (uint32_t)nx);
+    z = (int)(t / (uint32_t)ny);
+    y = (int)(t - (uint32_t)z * (uint32_t)ny);
+}
+
+__device__ __forceinline__ uint32_t sidx(int x, int y, int z, int nx, int ny)
+{
+    return (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+}
+
+// Population i of component c pulled into site (x,y,z): the value at
+// (x-c_i, y-c_i, z-c_i).  With Lees-Edwards a source outside 0..nx-1 in x
+// reads the x-halo plane face_pc.
+template <typename R, int C, bool LE>
+__device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__ face_pc,
+                                  const Phys<R> &P, int c, int i, int x, int y, int z)
+{
+    int xs = x - CX[i], ys = wrap1(y - CY[i], P.ny), zs = wrap1(z - CZ[i], P.nz);
+    if (LE) {
+        if (xs < 0 || xs >= P.nx) {
+            int fc = xs < 0 ? 0 : 1;
+            uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+            return face_pc[((uint32_t)(fc * C + c) * 19u + (uint32_t)i) * plane +
+                           (uint32_t)zs * (uint32_t)P.ny + (uint32_t)ys];
+        }
+    } else {
+        xs = wrap1(xs, P.nx);
+    }
+    return f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + sidx(xs, ys, zs, P.nx, P.ny)];
+}
+
+// Value k of the aux field at the neighbour (x+dx, y+dy, z+dz); for LE the
+// x-halo neighbours read face_aux.
+template <typename R, bool LE>
+__device__ __forceinline__ R aux_at(const R *__restrict__ aux, const R *__restrict__ face_aux,
+                                    const Phys<R> &P, int k, int x, int y, int z)
+{
+    int ys = wrap1(y, P.ny), zs = wrap1(z, P.nz);
+    if (LE) {
+        if (x < 0 || x >= P.nx) {
+            int fc = x < 0 ? 0 : 1;
+            uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+            return face_aux[(uint32_t)(fc * 4 + k) * plane + (uint32_t)zs * (uint32_t)P.ny +
+                            (uint32_t)ys];
+        }
+    } else {
+        x = wrap1(x, P.nx);
+    }
+    return aux[(uint32_t)k * P.ns + sidx(x, ys, zs, P.nx, P.ny)];
+}
+
+// psi(rho) = 1 - exp(-rho) (get_psi, code/lb3d_force_shanchen.F90:334-353);
+// deviation form: psi(rho0+d) - psi(rho0) = -exp(-rho0)*expm1(-d).
+template <bool Dev, typename R>
+__device__ __forceinline__ R psi_of(R rho, R emr0)
+{
+    if constexpr (Dev)
+        return -emr0 * expm1f(-rho);
+    else
+        return R(1.0) - exp(-rho);
+}
+
+// --------------------------------------------------------------------------
+// k_moments: streaming (pull) and the moments of the streamed populations,
+// code/lb3d_advection.F90:54-151 with lb3d_update_density
+// (code/lb3d_analysis.F90:64-76); ternary also the advected dipole
+// da = sum_i g_i n_s,i(x-c_i) db(x-c_i) (:130-148).  The streamed
+// populations themselves are not stored: k_collide pulls them again.
+template <typename R, bool Dev, int C, bool LE>
+__global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
+                                                 const R *__restrict__ face_pc,
+                                                 const R *__restrict__ db, R *__restrict__ aux)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R rho[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) rho[c] = R(0);
+    R da0 = R(0), da1 = R(0), da2 = R(0);
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            R v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+            rho[c] = rho[c] + v * R(GD[i]);
+            if constexpr (C == 3) {
+                if (c == 2) {
+                    R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
+                    R ng = ns_ * R(GD[i]);
+                    int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny),
+                        zs = wrap1(z - CZ[i], P.nz);
+                    uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
+                    da0 = da0 + ng * db[q];
+                    da1 = da1 + ng * db[P.ns + q];
+                    da2 = da2 + ng * db[2u * P.ns + q];
+                }
+            }
+        }
+    }
+    constexpr int NA = C == 3 ? NAUX3 : NAUX2;
+    (void)NA;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        aux[(uint32_t)c * P.ns + s] = psi_of<Dev>(rho[c], P.emr0[c]);
+        aux[(uint32_t)(C + c) * P.ns + s] = rho[c];
+    }
+    if constexpr (C == 3) {
+        aux[6u * P.ns + s] = da0;
+        aux[7u * P.ns + s] = da1;
+        aux[8u * P.ns + s] = da2;
+    }
+}
+
+// --------------------------------------------------------------------------
+// k_dipole: dipole relaxation, code/lb3d_collision.F90:296-403 (ternary).
+template <typename R, bool Dev>
+__global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__ aux,
+                                                R *__restrict__ db)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R hs0 = aux[6u * P.ns + s], hs1 = aux[7u * P.ns + s], hs2 = aux[8u * P.ns + s];
+    R rho_s = aux[5u * P.ns + s];
+    if constexpr (Dev) rho_s = P.rho0[2] + rho_s;
+    R sN = rho_s > R(CLAMP_SURF) ? rho_s : R(CLAMP_SURF);
+    R sd0 = hs0 / sN, sd1 = hs1 / sN, sd2 = hs2 / sN;
+    R hc0 = R(0), hc1 = R(0), hc2 = R(0);
+#pragma unroll
+    for (int i = 0; i < 18; i++) {
+        int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
+        // -sign(1,g_bs)*(rho_b - rho_r)(x+c_i)*g_i (raw densities, :361-369)
+        R rr = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
+        R rb = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
+        R gs = R(GD[i]);
+        R hc_m = -((P.sign_g_bs * (rb - rr)) * gs);
+        if (CX[i] != 0) hc0 = hc0 + hc_m * R(CX[i]);
+        if (CY[i] != 0) hc1 = hc1 + hc_m * R(CY[i]);
+        if (CZ[i] != 0) hc2 = hc2 + hc_m * R(CZ[i]);
+        R p0 = aux_at<R, false>(aux, nullptr, P, 6, xa, ya, za);
+        R p1 = aux_at<R, false>(aux, nullptr, P, 7, xa, ya, za);
+        R p2 = aux_at<R, false>(aux, nullptr, P, 8, xa, ya, za);
+        R hs_dot = R(2.0) * (R(CX[i]) * p0 + R(CY[i]) * p1 + R(CZ[i]) * p2);
+        hs0 = hs0 + (p0 - R(CX[i]) * hs_dot) * gs;
+        hs1 = hs1 + (p1 - R(CY[i]) * hs_dot) * gs;
+        hs2 = hs2 + (p2 - R(CZ[i]) * hs_dot) * gs;
+    }
+    R h0 = hc0 + hs0, h1 = hc1 + hs1, h2 = hc2 + hs2;
+    R h_m = P.beta * sqrt(h0 * h0 + h1 * h1 + h2 * h2);
+    R e0 = P.third_beta * h0, e1 = P.third_beta * h1, e2 = P.third_beta * h2;
+    if (h_m > R(10.0e-3)) {
+        R exh = R(1.0) / tanh(h_m);
+        R sd_mag = exh - R(1.0) / h_m;
+        e0 = ((P.beta * h0) / h_m) * sd_mag;
+        e1 = ((P.beta * h1) / h_m) * sd_mag;
+        e2 = ((P.beta * h2) / h_m) * sd_mag;
+    }
+    e0 = e0 * P.d_0;
+    e1 = e1 * P.d_0;
+    e2 = e2 * P.d_0;
+    db[s] = P.omega_d * e0 + P.one_m_omega_d * sd0;
+    db[P.ns + s] = P.omega_d * e1 + P.one_m_omega_d * sd1;
+    db[2u * P.ns + s] = P.omega_d * e2 + P.one_m_omega_d * sd2;
+}
+
+// --------------------------------------------------------------------------
+// Shan-Chen forces at site (x,y,z) from the aux field.
+// binary: code/lb3d_force_shanchen.F90:127-210; ternary: :215-330.
+template <typename R, bool Dev, int C, bool LE>
+__device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict__ aux,
+                                          const R *__restrict__ face_aux,
+                                          const R *__restrict__ db, int x, int y, int z,
+                                          uint32_t s, R F[C][3])
+{
+    R frb[3] = {R(0), R(0), R(0)}, fbr[3] = {R(0), R(0), R(0)};
+    if constexpr (C == 2) {
+#pragma unroll
+        for (int i = 0; i < 18; i++) {
+            int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
+            R pr = aux_at<R, LE>(aux, face_aux, P, 0, xa, ya, za);
+            R pb = aux_at<R, LE>(aux, face_aux, P, 1, xa, ya, za);
+            R green_r = pb * R(GD[i]);
+            R green_b = pr * R(GD[i]);
+            if (CX[i] != 0) { frb[0] = frb[0] + green_r * R(CX[i]); fbr[0] = fbr[0] + green_b * R(CX[i]); }
+            if (CY[i] != 0) { frb[1] = frb[1] + green_r * R(CY[i]); fbr[1] = fbr[1] + green_b * R(CY[i]); }
+            if (CZ[i] != 0) { frb[2] = frb[2] + green_r * R(CZ[i]); fbr[2] = fbr[2] + green_b * R(CZ[i]); }
+        }
+        R psr = aux[s], psb = aux[P.ns + s];
+        if constexpr (Dev) {
+            psr = P.psi0[0] + psr;
+            psb = P.psi0[1] + psb;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            F[1][k] = (-(P.g_br * fbr[k])) * psb;
+            F[0][k] = (-(P.g_br * frb[k])) * psr;
+        }
+    } else {
+        R fcs[3] = {R(0), R(0), R(0)}, fsc[3] = {R(0), R(0), R(0)}, fss[3] = {R(0), R(0), R(0)};
+        R dbx0 = db[s], dbx1 = db[P.ns + s], dbx2 = db[2u * P.ns + s];
+        const R dpsi_br = Dev ? P.psi0[1] - P.psi0[0] : R(0);
+#pragma unroll
+        for (int i = 0; i < 18; i++) {
+            int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
+            const R c0 = R(CX[i]), c1 = R(CY[i]), c2 = R(CZ[i]);
+            const R gs = R(GD[i]);
+            R pr = aux_at<R, false>(aux, nullptr, P, 0, xa, ya, za);
+            R pb = aux_at<R, false>(aux, nullptr, P, 1, xa, ya, za);
+            R ps = aux_at<R, false>(aux, nullptr, P, 2, xa, ya, za);
+            if constexpr (Dev) ps = P.psi0[2] + ps;
+            R green_r = pb * gs, green_b = pr * gs;
+            if (CX[i] != 0) { frb[0] = frb[0] + green_r * c0; fbr[0] = fbr[0] + green_b * c0; }
+            if (CY[i] != 0) { frb[1] = frb[1] + green_r * c1; fbr[1] = fbr[1] + green_b * c1; }
+            if (CZ[i] != 0) { frb[2] = frb[2] + green_r * c2; fbr[2] = fbr[2] + green_b * c2; }
+            R d0 = aux_at<R, false>(db, nullptr, P, 0, xa, ya, za) * ps;
+            R d1 = aux_at<R, false>(db, nullptr, P, 1, xa, ya, za) * ps;
+            R d2 = aux_at<R, false>(db, nullptr, P, 2, xa, ya, za) * ps;
+            R d_dotc = d0 * c0 + d1 * c1 + d2 * c2;
+            fcs[0] = fcs[0] + (d0 - (R(2.0) * d_dotc) * c0) * gs;
+            fcs[1] = fcs[1] + (d1 - (R(2.0) * d_dotc) * c1) * gs;
+            fcs[2] = fcs[2] + (d2 - (R(2.0) * d_dotc) * c2) * gs;
+            R di_dotc = dbx0 * c0 + dbx1 * c1 + dbx2 * c2;
+            R greenc_s = ((pb - pr) + dpsi_br) * gs;
+            fsc[0] = fsc[0] + greenc_s * (dbx0 - (R(2.0) * di_dotc) * c0);
+            fsc[1] = fsc[1] + greenc_s * (dbx1 - (R(2.0) * di_dotc) * c1);
+            fsc[2] = fsc[2] + greenc_s * (dbx2 - (R(2.0) * di_dotc) * c2);
+            R di_dot_d = d0 * dbx0 + d1 * dbx1 + d2 * dbx2;
+            R green_ss = R(8.0) * gs;
+            R a = di_dot_d - (R(2.0) * di_dotc) * d_dotc;
+            fss[0] = fss[0] + green_ss * ((a * c0 + d0 * di_dotc) + d_dotc * dbx0);
+            fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
+            fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
+        }
+        R psr = aux[s], psb = aux[P.ns + s], pss = aux[2u * P.ns + s];
+        if constexpr (Dev) {
+            psr = P.psi0[0] + psr;
+            psb = P.psi0[1] + psb;
+            pss = P.psi0[2] + pss;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            R f_br = -(P.g_br * fbr[k]);
+            R f_rb = -(P.g_br * frb[k]);
+            R f_sc = P.two_g_bs * fsc[k];
+            R f_cs = -(P.two_g_bs * fcs[k]);
+            R f_ss = -(P.g_ss * fss[k]);
+            F[1][k] = (f_br + f_cs) * psb;
+            F[0][k] = (f_rb - f_cs) * psr;
+            F[2][k] = (f_sc + f_ss) * pss;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// k_collide: pull, moments, Shan-Chen forces, body force, BGK relaxation
+// towards rho*boltz(u_tilde, du), code/lb3d_collision.F90:151-289 with
+// lb3d_get_sc_u_eq (:560-587) and lb3d_get_sc_du (:523-556).  Writes the
+// post-collision populations of the site into fout.
+template <typename R, bool Dev, int C, bool LE>
+__global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
+                                                 const R *__restrict__ face_pc,
+                                                 const R *__restrict__ aux,
+                                                 const R *__restrict__ face_aux,
+                                                 const R *__restrict__ db, R *__restrict__ fout)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R n[C][19];
+    R rho[C], mom[C][3];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        rho[c] = R(0);
+        mom[c][0] = mom[c][1] = mom[c][2] = R(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            R v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+            n[c][i] = v;
+            rho[c] = rho[c] + v * R(GD[i]);
+            R ng = v * R(GD[i]);
+            if (CX[i] != 0) mom[c][0] = mom[c][0] + ng * R(CX[i]);
+            if (CY[i] != 0) mom[c][1] = mom[c][1] + ng * R(CY[i]);
+            if (CZ[i] != 0) mom[c][2] = mom[c][2] + ng * R(CZ[i]);
+        }
+    }
+    R F[C][3];
+    sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+    R rhof[C];  // full densities
+#pragma unroll
+    for (int c = 0; c < C; c++) rhof[c] = Dev ? P.rho0[c] + rho[c] : rho[c];
+    if (P.body) {
+        int gx = x + 1, gy = y + 1, gz = z + 1;
+        if (gz >= P.bmin[2] && gz <= P.bmax[2] && gx >= P.bmin[0] && gx <= P.bmax[0] &&
+            gy >= P.bmin[1] && gy <= P.bmax[1]) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                R mass = P.amass[c] * rhof[c];
+                F[c][2] = F[c][2] + mass * P.g_accn[2];
+                F[c][1] = F[c][1] + mass * P.g_accn[1];
+                F[c][0] = F[c][0] + mass * P.g_accn[0];
+            }
+        }
+    }
+    // u_tilde = sum amass*omega*P / max(sum amass*rho*omega, 1e-8)
+    R pt0 = P.am_om[0] * mom[0][0], pt1 = P.am_om[0] * mom[0][1], pt2 = P.am_om[0] * mom[0][2];
+    R rt = (P.amass[0] * rhof[0]) * P.omega[0];
+#pragma unroll
+    for (int c = 1; c < C; c++) {
+        pt0 = pt0 + P.am_om[c] * mom[c][0];
+        pt1 = pt1 + P.am_om[c] * mom[c][1];
+        pt2 = pt2 + P.am_om[c] * mom[c][2];
+        rt = rt + (P.amass[c] * rhof[c]) * P.omega[c];
+    }
+    rt = rt > R(CLAMP_RHO) ? rt : R(CLAMP_RHO);
+    R ux = pt0 / rt, uy = pt1 / rt, uz = pt2 / rt;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        R pon = R(1.0) / (rhof[c] > R(CLAMP_RHO) ? rhof[c] : R(CLAMP_RHO));
+        R tp = P.tau[c] * pon;
+        R du0 = (tp * F[c][0]) / P.amass[c];
+        R du1 = (tp * F[c][1]) / P.amass[c];
+        R du2 = (tp * F[c][2]) / P.amass[c];
+        R b[19];
+        uint32_t base = (uint32_t)c * 19u * P.ns + s;
+        if constexpr (Dev) {
+            boltz_dev(P.bdist, ux, uy, uz, du0, du1, du2, b);
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                // deviation of rho*b_i from w_i*rho0: w_i*drho + rho*(b_i - w_i)
+                R wi = i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0);
+                R deq = wi * rho[c] + rhof[c] * b[i];
+                fout[base + (uint32_t)i * P.ns] = n[c][i] - P.omega[c] * (n[c][i] - deq);
+            }
+        } else {
+            boltz_dist<R>(P.bdist, ux, uy, uz, du0, du1, du2, b);
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                R neq = rho[c] * b[i];
+                fout[base + (uint32_t)i * P.ns] = n[c][i] - P.omega[c] * (n[c][i] - neq);
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Lees-Edwards x-halo planes, code/lb3d_bc_leesedwards.F90:659-1004 on one
+// rank (all partners are self).  For (y,z) of the halo plane, with
+// b = 0 (lelo >= 0) or -1 (lelo < 0) and f = lefrac:
+//   low  (x=0):    (1-f) N(nx, y, z+lelo+b)   + f N(nx, y, z+lelo+1+b)
+//   high (x=nx+1): f N(1, y, z-lelo-1-b)      + (1-f) N(1, y, z-lelo-b)
+// (z wrapped), then n += rho_c*(feq(u -+ 2U) - feq(u)) with the negativity
+// clamp; the high plane's clamp reads the finished low-plane site at the same
+// (y,z) (:951-990).  Post = false: the sources are the post-collision
+// populations (stage 6), written to face_pc.  Post = true: the sources are
+// the post-advection populations (pulled through face_pc, stage 8) and only
+// the halo moments are written to face_aux.
+template <typename R, bool Dev, bool Post>
+__device__ __forceinline__ R le_src(const Phys<R> &P, const R *__restrict__ f,
+                                    const R *__restrict__ face_pc, int c, int i, int x, int y, int z)
+{
+    if constexpr (Post)
+        return pull<R, 2, true>(f, face_pc, P, c, i, x, y, z);
+    else
+        return f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + sidx(x, y, z, P.nx, P.ny)];
+}
+
+template <typename R, bool Dev>
+__device__ __forceinline__ void le_correct(const Phys<R> &P, const LEState &L, R n[2][19],
+                                           const R base[2][19], bool low)
+{
+    // moments of the interpolated halo site
+    R rN[2], p[2][3];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        rN[c] = R(0);
+        p[c][0] = p[c][1] = p[c][2] = R(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            rN[c] = rN[c] + n[c][i] * R(GD[i]);
+            R ng = n[c][i] * R(GD[i]);
+            if (CX[i] != 0) p[c][0] = p[c][0] + ng * R(CX[i]);
+            if (CY[i] != 0) p[c][1] = p[c][1] + ng * R(CY[i]);
+            if (CZ[i] != 0) p[c][2] = p[c][2] + ng * R(CZ[i]);
+        }
+    R rf[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) rf[c] = Dev ? P.rho0[c] + rN[c] : rN[c];
+    R pt0 = P.am_om[0] * p[0][0], pt1 = P.am_om[0] * p[0][1], pt2 = P.am_om[0] * p[0][2];
+    R rt = (P.amass[0] * rf[0]) * P.omega[0];
+    pt0 = pt0 + P.am_om[1] * p[1][0];
+    pt1 = pt1 + P.am_om[1] * p[1][1];
+    pt2 = pt2 + P.am_om[1] * p[1][2];
+    rt = rt + (P.amass[1] * rf[1]) * P.omega[1];
+    rt = rt > R(CLAMP_RHO) ? rt : R(CLAMP_RHO);
+    R ux = pt0 / rt, uy = pt1 / rt, uz = pt2 / rt;
+    R su = R(L.shear_u);
+    R uzs = low ? (uz - su) - su : (uz + su) + su;
+    R b1[19], b2[19];
+    if constexpr (Dev) {
+        boltz_dev(P.bdist, ux, uy, uzs, R(0), R(0), R(0), b1);
+        boltz_dev(P.bdist, ux, uy, uz, R(0), R(0), R(0), b2);
+    } else {
+        boltz_dist<R>(P.bdist, ux, uy, uzs, R(0), R(0), R(0), b1);
+        boltz_dist<R>(P.bdist, ux, uy, uz, R(0), R(0), R(0), b2);
+    }
+    R df[2][19];
+    bool neg = false;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        R bd = b1[i] - b2[i];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            df[c][i] = rf[c] * bd;
+            R nn = n[c][i] + df[c][i];
+            R full = Dev ? (i < 18 ? P.wrho0_move[c] : P.wrho0_rest[c]) + nn : nn;
+            if (full < R(0)) neg = true;
+        }
+    }
+    if (neg) {
+        bool allpos = true;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                R full = Dev ? (i < 18 ? P.wrho0_move[c] : P.wrho0_rest[c]) + base[c][i] : base[c][i];
+                if (!(full > R(0))) allpos = false;
+            }
+        R scale = R(0);
+        if (allpos) {
+            bool first = true;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int i = 0; i < 19; i++) {
+                    R full = Dev ? (i < 18 ? P.wrho0_move[c] : P.wrho0_rest[c]) + base[c][i] : base[c][i];
+                    R v = -df[c][i] / full;
+                    if (first || v > scale) scale = v;
+                    first = false;
+                }
+        }
+        bool use = fabs(scale) > R(1.0);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) n[c][i] = use ? base[c][i] + df[c][i] / scale : base[c][i];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) n[c][i] = n[c][i] + df[c][i];
+    }
+}
+
+__device__ __forceinline__ int wrapz1(int k, int n)
+{  // k in 1-based terms, result 0-based index of wrap over 1..n
+    int m = (k - 1) % n;
+    if (m < 0) m += n;
+    return m;
+}
+
+template <typename R, bool Dev, bool Post>
+__global__ __launch_bounds__(128) void k_le_faces(Phys<R> P, const LEState *__restrict__ Lp,
+                                                  const R *__restrict__ f,
+                                                  R *__restrict__ face_pc,
+                                                  R *__restrict__ face_aux)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+    if (t >= plane) return;
+    const LEState L = *Lp;
+    int y = (int)(t % (uint32_t)P.ny), z = (int)(t / (uint32_t)P.ny);  // 0-based
+    int z1 = z + 1;                                                    // 1-based
+    int lelo = L.lelo, bsh = lelo >= 0 ? 0 : -1;
+    R fr = R(L.lefrac), omf = R(1.0) - R(L.lefrac);
+    R lo[2][19], hi[2][19];
+    {
+        int za = wrapz1(z1 + lelo + bsh, P.nz), zb = wrapz1(z1 + lelo + 1 + bsh, P.nz);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, P.nx - 1, y, za);
+                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, P.nx - 1, y, zb);
+                lo[c][i] = omf * a + fr * b;
+            }
+        R base[2][19];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) base[c][i] = lo[c][i];
+        le_correct<R, Dev>(P, L, lo, base, true);
+    }
+    {
+        int za = wrapz1(z1 - lelo - 1 - bsh, P.nz), zb = wrapz1(z1 - lelo - bsh, P.nz);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, 0, y, za);
+                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, 0, y, zb);
+                hi[c][i] = fr * a + omf * b;
+            }
+        le_correct<R, Dev>(P, L, hi, lo, false);
+    }
+    if constexpr (!Post) {
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                face_pc[((uint32_t)(0 * 2 + c) * 19u + (uint32_t)i) * plane + t] = lo[c][i];
+                face_pc[((uint32_t)(1 * 2 + c) * 19u + (uint32_t)i) * plane + t] = hi[c][i];
+            }
+    } else {
+#pragma unroll
+        for (int fc = 0; fc < 2; fc++) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                R rho = R(0);
+#pragma unroll
+                for (int i = 0; i < 19; i++) rho = rho + (fc == 0 ? lo[c][i] : hi[c][i]) * R(GD[i]);
+                face_aux[(uint32_t)(fc * 4 + c) * plane + t] = psi_of<Dev>(rho, P.emr0[c]);
+                face_aux[(uint32_t)(fc * 4 + 2 + c) * plane + t] = rho;
+            }
+        }
+    }
+}
+
+// le_neighbours (code/lb3d_bc_leesedwards.F90:86-140, inv_fluid = 5) with the
+// stage-6 update shear_u = shear_tmp*cos(shear_omega*nt) (code/lb3d_bc.F90:110),
+// in FP64 on the device so that the step sequence can be replayed from a graph.
+static __global__ void k_le_neighbours(LEState *L, int nz, int nt_set)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    LEState s = *L;
+    s.nt = nt_set >= 0 ? nt_set : s.nt + 1;
+    s.shear_u = s.shear_tmp * cos(s.shear_omega * (double)s.nt);
+    s.shear_sum = s.shear_sum + s.shear_u;
+    // lelo = mod(int(2.*shear_sum), nz): truncation, sign of the dividend
+    long t = (long)(2.0 * s.shear_sum);
+    s.lelo = (int)(t % (long)nz);
+    double lf = 2.0 * s.shear_sum;
+    s.lefrac = fabs(lf - trunc(lf));
+    *L = s;
+}
+
+}  // namespace lb3d

@@ -1,0 +1,152 @@
+// lb3d_lattice.h -- D3Q19 constants, the equilibrium distribution and the
+// kernel-side parameter block shared by all LB3D kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lb3d {
+
+// D3Q19 vectors (index 18 = rest) and the D3Q25 projection degeneracies g,
+// code/lb3d_lattice.F90:137-164.  g also serves as the Shan-Chen stencil
+// weight g_sc (code/lb3d_force_shanchen.F90:145-149).
+__device__ constexpr int CX[19] = {1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+__device__ constexpr int CY[19] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0};
+__device__ constexpr int CZ[19] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0};
+__device__ constexpr int GD[19] = {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+
+// Host copies of the same tables.
+namespace host {
+constexpr int CX[19] = {1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+constexpr int CY[19] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0};
+constexpr int CZ[19] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0};
+constexpr int GD[19] = {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+}  // namespace host
+
+// Float literals of the reference widened to double: dble(10.e-9) and
+// dble(10.e-5) (code/lb3d_collision.F90:345, 546, 586).
+constexpr double CLAMP_RHO = 9.99999993922529e-09;
+constexpr double CLAMP_SURF = 9.999999747378752e-05;
+
+// Kernel-side parameters, converted once on the host into the working type R.
+template <typename R>
+struct Phys {
+    int nx, ny, nz;
+    uint32_t ns;          // sites nx*ny*nz
+    int bdist;
+    int body;             // body force box active
+    int bmin[3], bmax[3]; // 1-based inclusive box (x, y, z)
+    R amass[3], tau[3], omega[3], am_om[3];  // am_om = amass*omega
+    R omega_d, one_m_omega_d, beta, third_beta, d_0;
+    R g_br, g_bs, g_ss, two_g_bs, sign_g_bs;
+    R g_accn[3];
+    // deviation storage: w_i*rho0 per component and psi(rho0), exp(-rho0)
+    R rho0[3], wrho0_move[3], wrho0_rest[3], psi0[3], emr0[3];
+};
+
+// Lees-Edwards scalars, updated on the device once per step (FP64).
+struct LEState {
+    double shear_u, shear_tmp, shear_sum, lefrac, shear_omega;
+    int lelo;
+    int nt;
+};
+
+// Equilibrium distribution, code/lb3d_bdist.F90:175-370 (COMPAT_BDIST):
+// bdist 0 = Luo (:214-234), 2 = third-order Taylor (:301-370), in the
+// reference's operation order.  b[i] are the weights w_i * (...), so that
+// the equilibrium population is rho*b[i].
+template <typename R>
+__device__ __forceinline__ void boltz_dist(int bdist, R ux, R uy, R uz, R dux, R duy, R duz,
+                                           R b[19])
+{
+    const R holdit = R(3.0);  // 1/T with T = 1/3
+    const R w = R(1.0 / 36.0);
+    if (bdist == 0) {
+        R uke = R(0.5) * (ux * ux + uy * uy + uz * uz);
+        R udotdu = ux * dux + uy * duy + uz * duz;
+        R uholdit = uke * holdit;
+#pragma unroll
+        for (int s = 0; s < 18; s++) {
+            R cdotu = R(CX[s]) * ux + R(CY[s]) * uy + R(CZ[s]) * uz;
+            R cdotdu = R(CX[s]) * dux + R(CY[s]) * duy + R(CZ[s]) * duz;
+            R ch = cdotu * holdit;
+            R cdh = cdotdu * holdit;
+            R ch2 = ch * ch;
+            b[s] = w * (R(1.0) + ch + R(0.5) * ch2 - uholdit + (ch * ch2) / R(6.0) - uholdit * ch +
+                        cdh + ch * cdh - udotdu * holdit + R(0.5) * ch2 * cdh -
+                        ch * udotdu * holdit - uholdit * cdh);
+        }
+        const R T = R(1.0 / 3.0);
+        b[18] = R(1.0 / 3.0) * (R(1.0) - uke / T - udotdu / T);
+        return;
+    }
+    R tux = ux + dux, tuy = uy + duy, tuz = uz + duz;
+    R tx = tux * holdit, ty = tuy * holdit, tz = tuz * holdit;
+    R uke = R(0.5) * (tux * tx + tuy * ty + tuz * tz);
+    R cst = R(1.0) - uke;
+    R c1 = w * cst;
+    const R wby2 = w / R(2.0), wby6 = w / R(6.0);
+    auto pair = [&](R cu, int ip, int im) {
+        R even = c1 + wby2 * (cu * cu);
+        R odd = c1 * cu + wby6 * (cu * cu * cu);
+        b[ip] = even + odd;
+        b[im] = even - odd;
+    };
+    pair(tx, 0, 1);
+    pair(ty, 2, 3);
+    pair(tz, 4, 5);
+    pair(tx + ty, 6, 11);
+    pair(tx - ty, 7, 10);
+    pair(tx + tz, 8, 13);
+    pair(tx - tz, 9, 12);
+    pair(ty + tz, 14, 17);
+    pair(ty - tz, 15, 16);
+    b[18] = R(1.0 / 3.0) * cst;
+}
+
+// Deviation form of the equilibrium for FP32 storage: bw[i] = b[i] - w_i,
+// computed without forming b (only small terms), bdist 2 and 0.
+__device__ __forceinline__ void boltz_dev(int bdist, float ux, float uy, float uz, float dux,
+                                          float duy, float duz, float bw[19])
+{
+    const float w = 1.0f / 36.0f;
+    if (bdist == 0) {
+        float uke = 0.5f * (ux * ux + uy * uy + uz * uz);
+        float udotdu = ux * dux + uy * duy + uz * duz;
+        float uh = 3.0f * uke;
+#pragma unroll
+        for (int s = 0; s < 18; s++) {
+            float ch = 3.0f * (float(CX[s]) * ux + float(CY[s]) * uy + float(CZ[s]) * uz);
+            float cdh = 3.0f * (float(CX[s]) * dux + float(CY[s]) * duy + float(CZ[s]) * duz);
+            float ch2 = ch * ch;
+            bw[s] = w * (ch + 0.5f * ch2 - uh + (ch * ch2) * (1.0f / 6.0f) - uh * ch + cdh +
+                         ch * cdh - 3.0f * udotdu + 0.5f * ch2 * cdh - 3.0f * ch * udotdu - uh * cdh);
+        }
+        bw[18] = (1.0f / 3.0f) * (-3.0f * uke - 3.0f * udotdu);
+        return;
+    }
+    float tx = 3.0f * (ux + dux), ty = 3.0f * (uy + duy), tz = 3.0f * (uz + duz);
+    float uke = (1.0f / 6.0f) * (tx * tx + ty * ty + tz * tz);
+    float wc = w * (1.0f - uke);
+    float wuke = -w * uke;
+    auto pair = [&](float cu, int ip, int im) {
+        float even = wuke + (0.5f * w) * (cu * cu);
+        float odd = wc * cu + (w / 6.0f) * (cu * cu * cu);
+        bw[ip] = even + odd;
+        bw[im] = even - odd;
+    };
+    pair(tx, 0, 1);
+    pair(ty, 2, 3);
+    pair(tz, 4, 5);
+    pair(tx + ty, 6, 11);
+    pair(tx - ty, 7, 10);
+    pair(tx + tz, 8, 13);
+    pair(tx - tz, 9, 12);
+    pair(ty + tz, 14, 17);
+    pair(ty - tz, 15, 16);
+    bw[18] = -(1.0f / 3.0f) * uke;
+}
+
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+}  // namespace lb3d

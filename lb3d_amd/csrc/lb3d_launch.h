@@ -1,0 +1,229 @@
+// lb3d_launch.h -- the Launcher implementation, included by exactly one
+// translation unit per storage form (lb3d_fp64.hip, lb3d_fp32.hip).
+#pragma once
+
+#include <cmath>
+
+#include "lb3d_kernels.h"
+#include "lb3d_solver.h"
+
+namespace lb3d {
+
+template <typename R>
+Phys<R> make_phys(const lb3d_params &p)
+{
+    Phys<R> P{};
+    P.nx = p.nx;
+    P.ny = p.ny;
+    P.nz = p.nz;
+    P.ns = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+    P.bdist = p.bdist;
+    P.body = (p.g_accn[0] != 0.0 || p.g_accn[1] != 0.0 || p.g_accn[2] != 0.0) ? 1 : 0;
+    const int ext[3] = {p.nx, p.ny, p.nz};
+    for (int k = 0; k < 3; k++) {
+        P.bmin[k] = p.g_accn_min[k];
+        P.bmax[k] = p.g_accn_max[k] == 0 ? ext[k] : p.g_accn_max[k];
+    }
+    for (int c = 0; c < 3; c++) {
+        double om = 1.0 / p.tau[c];
+        P.amass[c] = R(p.amass[c]);
+        P.tau[c] = R(p.tau[c]);
+        P.omega[c] = R(om);
+        P.am_om[c] = R(p.amass[c] * om);
+        P.g_accn[c] = R(p.g_accn[c]);
+        double r0 = p.rho0[c] > 0.0 ? p.rho0[c] : 0.7;
+        P.rho0[c] = R(r0);
+        P.wrho0_move[c] = R(r0 / 36.0);
+        P.wrho0_rest[c] = R(r0 / 3.0);
+        P.psi0[c] = R(1.0 - std::exp(-r0));
+        P.emr0[c] = R(std::exp(-r0));
+    }
+    double omd = 1.0 / p.tau_d;
+    P.omega_d = R(omd);
+    P.one_m_omega_d = R(1.0 - omd);
+    P.beta = R(p.beta);
+    P.third_beta = R((1.0 / 3.0) * p.beta);
+    P.d_0 = R(p.d_0);
+    P.g_br = R(p.g_br);
+    P.g_bs = R(p.g_bs);
+    P.g_ss = R(p.g_ss);
+    P.two_g_bs = R(2.0 * p.g_bs);
+    P.sign_g_bs = R(std::signbit(p.g_bs) ? -1.0 : 1.0);
+    return P;
+}
+
+// Seeded synthetic state (shares the hash of the oracle's or_init_noise).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double hash_uniform(uint64_t seed, uint32_t stream, uint64_t s)
+{
+    uint64_t h = mix64(mix64(seed ^ ((uint64_t)stream << 48)) ^ s);
+    return (double)(h >> 11) * 0x1.0p-53;
+}
+
+template <typename R, bool Dev, int C>
+__global__ __launch_bounds__(256) void k_init_noise(Phys<R> P, uint64_t seed, double m0, double m1,
+                                                    double m2, double a0, double a1, double a2,
+                                                    double r00, double r01, double r02, int dipoles,
+                                                    R *__restrict__ f, R *__restrict__ db)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    const double mean[3] = {m0, m1, m2}, amp[3] = {a0, a1, a2}, r0[3] = {r00, r01, r02};
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        double u = hash_uniform(seed, (uint32_t)c, (uint64_t)s);
+        double rho = mean[c] + amp[c] * (2.0 * u - 1.0);
+        R mv = Dev ? R((1.0 / 36.0) * (rho - r0[c])) : R((1.0 / 36.0) * rho);
+        R rv = Dev ? R((1.0 / 3.0) * (rho - r0[c])) : R((1.0 / 3.0) * rho);
+        for (int i = 0; i < 18; i++) f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + s] = mv;
+        f[((uint32_t)c * 19u + 18u) * P.ns + s] = rv;
+    }
+    if (C == 3) {
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        if (dipoles) {
+            double zc = 2.0 * hash_uniform(seed, 3u, (uint64_t)s) - 1.0;
+            double ph = 6.283185307179586 * hash_uniform(seed, 4u, (uint64_t)s);
+            double r = sqrt(1.0 - zc * zc);
+            d0 = r * cos(ph);
+            d1 = r * sin(ph);
+            d2 = zc;
+        }
+        db[s] = R(d0);
+        db[P.ns + s] = R(d1);
+        db[2u * P.ns + s] = R(d2);
+    }
+}
+
+// Sanity reductions (code/lb3d_sanity.F90:79-224) on the post-collision
+// populations: per-block FP64 partial sums, reduced on the host.
+template <typename R, bool Dev, int C>
+__global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restrict__ f,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double red[NDIAG][256];
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[NDIAG];
+    for (int k = 0; k < NDIAG; k++) v[k] = 0.0;
+    if (s < P.ns) {
+        bool isnan_ = false, isneg = false, fast = false;
+        for (int c = 0; c < C; c++) {
+            double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+            for (int i = 0; i < 19; i++) {
+                double n = (double)f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + s];
+                if (Dev) n = (i < 18 ? (double)P.rho0[c] / 36.0 : (double)P.rho0[c] / 3.0) + n;
+                if (n != n) isnan_ = true;
+                if (n < 0.0) isneg = true;
+                double ng = n * (double)GD[i];
+                rho = rho + ng;
+                m0 = m0 + ng * (double)CX[i];
+                m1 = m1 + ng * (double)CY[i];
+                m2 = m2 + ng * (double)CZ[i];
+            }
+            v[c] = rho;
+            double am = (double)P.amass[c];
+            v[3] = v[3] + am * m0;
+            v[4] = v[4] + am * m1;
+            v[5] = v[5] + am * m2;
+            if (fabs(rho) > 1.0e-12) {
+                double a0 = m0 * am / rho, a1 = m1 * am / rho, a2 = m2 * am / rho;
+                if (sqrt(a0 * a0 + a1 * a1 + a2 * a2) > 0.1) fast = true;
+            }
+        }
+        v[6] = isnan_ ? 1.0 : 0.0;
+        v[7] = isneg ? 1.0 : 0.0;
+        v[8] = fast ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < NDIAG; k++) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < NDIAG; k++) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NDIAG; k++) partials[(size_t)blockIdx.x * NDIAG + k] = red[k][0];
+}
+
+template <typename R, bool Dev>
+class LauncherT : public Launcher {
+  public:
+    explicit LauncherT(const lb3d_params &p)
+        : P(make_phys<R>(p)), C(p.ncomp), le(p.boundary == LB3D_BC_LEES_EDWARDS)
+    {
+    }
+    unsigned blocks() const { return (P.ns + BLOCK - 1) / BLOCK; }
+    unsigned diag_blocks() const override { return blocks(); }
+    size_t real_bytes() const override { return sizeof(R); }
+
+    void moments(const DevBufs &b, hipStream_t s) override
+    {
+        const R *f = (const R *)b.f[b.cur];
+        if (C == 3)
+            k_moments<R, Dev, 3, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux);
+        else if (le)
+            k_moments<R, Dev, 2, true><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux);
+        else
+            k_moments<R, Dev, 2, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux);
+    }
+    void dipole(const DevBufs &b, hipStream_t s) override
+    {
+        if (C == 3) k_dipole<R, Dev><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db);
+    }
+    void collide(const DevBufs &b, hipStream_t s) override
+    {
+        const R *f = (const R *)b.f[b.cur];
+        R *fo = (R *)b.f[1 - b.cur];
+        if (C == 3)
+            k_collide<R, Dev, 3, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                  (const R *)b.db, fo);
+        else if (le)
+            k_collide<R, Dev, 2, true><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, (const R *)b.aux,
+                                                                 (const R *)b.face_aux, nullptr, fo);
+        else
+            k_collide<R, Dev, 2, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                  nullptr, fo);
+    }
+    void le_faces(const DevBufs &b, bool post, hipStream_t s) override
+    {
+        unsigned plane = (unsigned)P.ny * (unsigned)P.nz;
+        unsigned nb = (plane + 127) / 128;
+        const R *f = (const R *)b.f[b.cur];
+        if (post)
+            k_le_faces<R, Dev, true><<<nb, 128, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+        else
+            k_le_faces<R, Dev, false><<<nb, 128, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+    }
+    void init_noise(const DevBufs &b, uint64_t seed, const double m[3], const double a[3], int dip,
+                    hipStream_t s) override
+    {
+        R *f = (R *)b.f[b.cur];
+        double r0[3] = {(double)P.rho0[0], (double)P.rho0[1], (double)P.rho0[2]};
+        if (C == 3)
+            k_init_noise<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, seed, m[0], m[1], m[2], a[0], a[1], a[2], r0[0],
+                                                                r0[1], r0[2], dip, f, (R *)b.db);
+        else
+            k_init_noise<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, seed, m[0], m[1], m[2], a[0], a[1], a[2], r0[0],
+                                                                r0[1], r0[2], dip, f, nullptr);
+    }
+    void diagnostics(const DevBufs &b, double *partials, hipStream_t s) override
+    {
+        const R *f = (const R *)b.f[b.cur];
+        if (C == 3)
+            k_diagnostics<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+        else
+            k_diagnostics<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+    }
+
+  private:
+    Phys<R> P;
+    int C;
+    bool le;
+};
+
+}  // namespace lb3d

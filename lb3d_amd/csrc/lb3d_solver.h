@@ -1,0 +1,47 @@
+// lb3d_solver.h -- host-side interface between the C ABI (lb3d_gpu.hip) and
+// the precision-specific kernel launchers (lb3d_fp64.hip, lb3d_fp32.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/lb3d_gpu.h"
+#include "lb3d_lattice.h"
+
+namespace lb3d {
+
+// Device buffers of one lattice.  f[cur] holds the post-collision populations
+// of step nt; collide writes f[1-cur].
+struct DevBufs {
+    void *f[2] = {nullptr, nullptr};
+    void *aux = nullptr, *db = nullptr, *face_pc = nullptr, *face_aux = nullptr;
+    LEState *le = nullptr;
+    int cur = 0;
+};
+
+// One precision's kernels for one parameter set.
+class Launcher {
+  public:
+    virtual ~Launcher() {}
+    virtual void moments(const DevBufs &b, hipStream_t s) = 0;
+    virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
+    virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
+    virtual void le_faces(const DevBufs &b, bool post, hipStream_t s) = 0;
+    virtual void init_noise(const DevBufs &b, uint64_t seed, const double rho_mean[3],
+                            const double amp[3], int dipoles, hipStream_t s) = 0;
+    // per-block FP64 partial sums for the sanity reductions (NDIAG per block)
+    virtual void diagnostics(const DevBufs &b, double *partials, hipStream_t s) = 0;
+    virtual unsigned diag_blocks() const = 0;
+    virtual size_t real_bytes() const = 0;
+};
+
+constexpr int NDIAG = 9;  // mass[3], momentum[3], nan sites, negative sites, fast sites
+constexpr unsigned BLOCK = 256;
+
+Launcher *make_launcher_fp64(const lb3d_params &p);
+Launcher *make_launcher_fp32(const lb3d_params &p);
+
+// Parameters in the kernels' working type.
+template <typename R>
+Phys<R> make_phys(const lb3d_params &p);
+
+}  // namespace lb3d

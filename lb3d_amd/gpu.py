@@ -1,0 +1,251 @@
+"""Python host for the MI355X LB3D timestep library (ctypes over the C ABI of
+include/lb3d_gpu.h).
+
+`Lattice` mirrors the reference's Fortran call surface for one lattice:
+  bc_init()            lb3d_bc_init(4)           code/lb3d_bc.F90:48-90
+  force_apply(stage)   lb3d_force_apply(stage)   code/lb3d_force.F90:98-121
+  bc_apply(stage)      lb3d_bc_apply(stage)      code/lb3d_bc.F90:95-155
+  advection()          lb3d_advection()          code/lb3d_advection.F90:40-50
+  collision()          lb3d_collision()          code/lb3d_collision.F90:41-47
+  time_loop(n)         lb3d_time_loop()          code/lb3d_time_loop.F90:42-97
+  sanity_check()       lbe_sanity_check(N, ...)  code/lb3d_sanity.F90:79-224
+Errors raise LB3DError (the reference aborts through abend/MPI_Abort).  The
+library is required: there is no CPU fallback, and a missing or unloadable
+liblb3d_gpu.so raises at import of the binding.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblb3d_gpu.so")
+
+LB3D_FP64 = 0
+LB3D_FP32_DEV = 1
+LB3D_BC_PERIODIC = 0
+LB3D_BC_LEES_EDWARDS = 1
+ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ESTATE", 4: "ENOMEM"}
+
+# Every entry point declared in include/lb3d_gpu.h.
+EXPORTS = [
+    "lb3d_gpu_abi_version", "lb3d_gpu_last_error", "lb3d_gpu_create", "lb3d_gpu_destroy",
+    "lb3d_gpu_upload_sites", "lb3d_gpu_download_sites", "lb3d_gpu_init_noise",
+    "lb3d_gpu_bc_init", "lb3d_gpu_force_apply", "lb3d_gpu_bc_apply", "lb3d_gpu_advection",
+    "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
+    "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
+    "lb3d_gpu_get_density", "lb3d_gpu_synchronize",
+]
+
+
+class LB3DError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """lb3d_params of include/lb3d_gpu.h."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("ncomp", ctypes.c_int32), ("bdist", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("boundary", ctypes.c_int32), ("restore", ctypes.c_int32),
+        ("amass", ctypes.c_double * 3), ("tau", ctypes.c_double * 3),
+        ("tau_d", ctypes.c_double), ("beta", ctypes.c_double), ("d_0", ctypes.c_double),
+        ("g_br", ctypes.c_double), ("g_bs", ctypes.c_double), ("g_ss", ctypes.c_double),
+        ("shear_u", ctypes.c_double), ("shear_omega", ctypes.c_double),
+        ("shear_sum", ctypes.c_double),
+        ("g_accn", ctypes.c_double * 3),
+        ("g_accn_min", ctypes.c_int32 * 3), ("g_accn_max", ctypes.c_int32 * 3),
+        ("rho0", ctypes.c_double * 3),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python -m lb3d_amd.build "
+                               "(hipcc, gfx950); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)
+        L.lb3d_gpu_last_error.restype = ctypes.c_char_p
+        L.lb3d_gpu_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.lb3d_gpu_destroy.argtypes = [vp]
+        L.lb3d_gpu_upload_sites.argtypes = [vp, dp, ip, ip, ip]
+        L.lb3d_gpu_download_sites.argtypes = [vp, dp, ip, ip]
+        L.lb3d_gpu_init_noise.argtypes = [vp, ctypes.c_uint64, dp, dp, ip]
+        L.lb3d_gpu_bc_init.argtypes = [vp]
+        L.lb3d_gpu_force_apply.argtypes = [vp, ip]
+        L.lb3d_gpu_bc_apply.argtypes = [vp, ip, ip]
+        L.lb3d_gpu_advection.argtypes = [vp]
+        L.lb3d_gpu_collision.argtypes = [vp]
+        L.lb3d_gpu_run_steps.argtypes = [vp, ip]
+        L.lb3d_gpu_get_nt.argtypes = [vp, ctypes.POINTER(ip)]
+        L.lb3d_gpu_diagnostics.argtypes = [vp, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+        L.lb3d_gpu_le_state.argtypes = [vp, dp, ctypes.POINTER(ip), dp, dp]
+        L.lb3d_gpu_get_populations.argtypes = [vp, dp]
+        L.lb3d_gpu_get_density.argtypes = [vp, dp]
+        L.lb3d_gpu_synchronize.argtypes = [vp]
+        L.lb3d_gpu_profile_steps.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_char_p), dp,
+                                             ctypes.POINTER(ip), dp]
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != 0:
+        raise LB3DError(rc, where, lib().lb3d_gpu_last_error().decode())
+
+
+def make_params(nx, ny, nz, ncomp=2, bdist=2, precision=LB3D_FP64, boundary=LB3D_BC_PERIODIC,
+                restore=0, amass=(1.0, 1.0, 1.0), tau=(1.0, 1.0, 1.0), tau_d=1.0, beta=10.0,
+                d_0=1.0, g_br=0.08, g_bs=-0.006, g_ss=-0.003, shear_u=0.0, shear_omega=0.0,
+                shear_sum=0.0, g_accn=(0.0, 0.0, 0.0), g_accn_min=(0, 0, 0),
+                g_accn_max=(0, 0, 0), rho0=(0.7, 0.7, 0.7)):
+    p = Params()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.ncomp, p.bdist, p.precision, p.boundary, p.restore = ncomp, bdist, precision, boundary, restore
+    p.amass[:] = list(map(float, amass))
+    p.tau[:] = list(map(float, tau))
+    p.tau_d, p.beta, p.d_0 = float(tau_d), float(beta), float(d_0)
+    p.g_br, p.g_bs, p.g_ss = float(g_br), float(g_bs), float(g_ss)
+    p.shear_u, p.shear_omega, p.shear_sum = float(shear_u), float(shear_omega), float(shear_sum)
+    p.g_accn[:] = list(map(float, g_accn))
+    p.g_accn_min[:] = list(map(int, g_accn_min))
+    p.g_accn_max[:] = list(map(int, g_accn_max))
+    p.rho0[:] = list(map(float, rho0))
+    return p
+
+
+class Lattice:
+    """One lattice on the GPU behind the reference's call surface."""
+
+    def __init__(self, nx, ny, nz, **kw):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.params = make_params(nx, ny, nz, **kw)
+        self.ncomp = self.params.ncomp
+        self.site_doubles = 79 if self.ncomp == 3 else 49
+        h = ctypes.c_void_p()
+        check(lib().lb3d_gpu_create(ctypes.byref(self.params), ctypes.byref(h)), "lb3d_gpu_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lb3d_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state ---------------------------------------------------------
+    def upload_sites(self, whole_N, halo_extent=1, nt0=0):
+        a = np.ascontiguousarray(whole_N, dtype=np.float64)
+        check(lib().lb3d_gpu_upload_sites(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          halo_extent, self.site_doubles, nt0), "upload_sites")
+
+    def download_sites(self, halo_extent=1):
+        h = halo_extent
+        out = np.zeros((self.nz + 2 * h, self.ny + 2 * h, self.nx + 2 * h, self.site_doubles))
+        check(lib().lb3d_gpu_download_sites(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            h, self.site_doubles), "download_sites")
+        return out
+
+    def init_noise(self, seed, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True):
+        m = (ctypes.c_double * 3)(*rho_mean)
+        a = (ctypes.c_double * 3)(*amp)
+        check(lib().lb3d_gpu_init_noise(self._h, ctypes.c_uint64(seed), m, a, 1 if dipoles else 0),
+              "init_noise")
+
+    def populations(self):
+        """Interior populations in FP64, shape (ncomp, 19, nz, ny, nx)."""
+        out = np.empty((self.ncomp, 19, self.nz, self.ny, self.nx))
+        check(lib().lb3d_gpu_get_populations(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "get_populations")
+        return out
+
+    def density(self):
+        """Interior densities rho_c in FP64, shape (ncomp, nz, ny, nx)."""
+        out = np.empty((self.ncomp, self.nz, self.ny, self.nx))
+        check(lib().lb3d_gpu_get_density(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "get_density")
+        return out
+
+    def synchronize(self):
+        check(lib().lb3d_gpu_synchronize(self._h), "synchronize")
+
+    @property
+    def nt(self):
+        v = ctypes.c_int32()
+        check(lib().lb3d_gpu_get_nt(self._h, ctypes.byref(v)), "get_nt")
+        return v.value
+
+    # --- the reference's call surface ----------------------------------
+    def bc_init(self):
+        check(lib().lb3d_gpu_bc_init(self._h), "bc_init")
+
+    def force_apply(self, stage):
+        check(lib().lb3d_gpu_force_apply(self._h, stage), "force_apply")
+
+    def bc_apply(self, stage, nt=None):
+        if nt is None:
+            nt = self.nt + 1 if stage == 6 else self.nt
+        check(lib().lb3d_gpu_bc_apply(self._h, stage, nt), "bc_apply")
+
+    def advection(self):
+        check(lib().lb3d_gpu_advection(self._h), "advection")
+
+    def collision(self):
+        check(lib().lb3d_gpu_collision(self._h), "collision")
+
+    def time_step(self):
+        """One timestep through the staged surface (code/lb3d_time_loop.F90:58-92)."""
+        nt = self.nt + 1
+        self.force_apply(6)
+        self.bc_apply(6, nt)
+        self.advection()
+        self.bc_apply(8, nt)
+        self.force_apply(8)
+        self.collision()
+
+    def time_loop(self, nsteps):
+        """nsteps timesteps replayed from a captured HIP graph."""
+        check(lib().lb3d_gpu_run_steps(self._h, int(nsteps)), "run_steps")
+
+    def sanity_check(self):
+        m = (ctypes.c_double * 3)()
+        p = (ctypes.c_double * 3)()
+        c = (ctypes.c_int64 * 3)()
+        check(lib().lb3d_gpu_diagnostics(self._h, m, p, c), "diagnostics")
+        return np.array(m[:]), np.array(p[:]), np.array(c[:])
+
+    def le_state(self):
+        s, f, u = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lo = ctypes.c_int32()
+        check(lib().lb3d_gpu_le_state(self._h, ctypes.byref(s), ctypes.byref(lo), ctypes.byref(f),
+                                      ctypes.byref(u)), "le_state")
+        return s.value, lo.value, f.value, u.value
+
+    def profile_steps(self, nsteps):
+        """Mean HIP-event duration per kernel launch over nsteps timesteps."""
+        mx = 8
+        names = (ctypes.c_char_p * mx)()
+        ms = (ctypes.c_double * mx)()
+        nk = ctypes.c_int32()
+        mps = ctypes.c_double()
+        check(lib().lb3d_gpu_profile_steps(self._h, int(nsteps), mx, names, ms, ctypes.byref(nk),
+                                           ctypes.byref(mps)), "profile_steps")
+        return {names[i].decode(): ms[i] for i in range(nk.value)}, mps.value

@@ -1,0 +1,51 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point that
+include/lb3d_gpu.h declares, and rejects invalid parameters before touching
+the device (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lb3d_amd import gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "lb3d_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(lb3d_gpu_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(gpu.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = gpu.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.lb3d_gpu_abi_version() == 1
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(ncomp=3, boundary=gpu.LB3D_BC_LEES_EDWARDS), "surfactant"),
+    (dict(ncomp=4), "ncomp"),
+    (dict(bdist=1), "bdist"),
+    (dict(precision=7), "precision"),
+    (dict(tau=(0.0, 1.0, 1.0)), "tau"),
+])
+def test_invalid_parameters_rejected(kw, msg):
+    with pytest.raises(gpu.LB3DError) as e:
+        gpu.Lattice(8, 8, 8, **kw)
+    assert e.value.code == 1 and msg in str(e.value)
+
+
+def test_too_large_lattice_rejected():
+    with pytest.raises(gpu.LB3DError):
+        gpu.Lattice(2048, 2048, 512, ncomp=3)
+
+
+def test_params_struct_layout():
+    # the Fortran bind(C) type and the C struct must agree on this size
+    assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3
