@@ -42,6 +42,9 @@ def run_form(n, ncomp, precision, steps, warmup, profile_steps):
     g = Lattice(n, n, n, ncomp=ncomp, precision=precision, g_br=0.08, bdist=2)
     g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
     g.bc_init()
+    report = g.autotune(trial_steps=3)
+    chosen = report.strip().splitlines()[-1]
+    print(f"[bench] autotune: {chosen}", file=sys.stderr, flush=True)
     g.time_loop(warmup)
     g.synchronize()
     t0 = time.perf_counter()
@@ -53,7 +56,7 @@ def run_form(n, ncomp, precision, steps, warmup, profile_steps):
     m, _, counts = g.sanity_check()
     g.close()
     return dict(seconds=dt, ms_per_step=1e3 * dt / steps, mlups=n ** 3 * steps / dt / 1e6,
-                kernels=kern, prof_ms_per_step=ms_step_prof, rho=rho, mass=m.tolist(),
+                kernels=kern, plan=chosen, prof_ms_per_step=ms_step_prof, rho=rho, mass=m.tolist(),
                 counts=counts.tolist())
 
 
@@ -82,7 +85,20 @@ def main():
     ap.add_argument("--cpu-sample-size", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (a kernel of known "
+                         "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path is built and measured on a single MI355X)", file=sys.stderr)
@@ -108,16 +124,17 @@ def main():
     ns = n ** 3
     B32 = bytes_per_site(ncomp, False, 4)
     B64 = bytes_per_site(ncomp, False, 8)
-    kc = r32["kernels"]["collide"]
+    kname = "fused" if "fused" in r32["kernels"] else "collide"
+    kc = r32["kernels"][kname]
     achieved = B32 * ns / (kc * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_collide (FP32 deviation)",
+                    kernel=f"k_{'fused2' if kname == 'fused' else 'collide'} (FP32 deviation), plan {r32['plan']}",
                     algorithmic_bytes_per_launch=B32 * ns, mean_launch_ms=kc,
                     step_frac=round(r32["mlups"] * 1e6 * B32 / (HBM_PEAK_GBS * 1e9), 4))
     fp64 = dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
                 step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
-                kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()})
+                kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()}, plan=r64["plan"])
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline oracle {args.cpu_sample_size}^3 x{args.cpu_sample_steps}", file=sys.stderr, flush=True)

@@ -129,6 +129,16 @@ int lb3d_gpu_collision(lb3d_ctx *ctx);                           /* code/lb3d_co
  * code/lb3d_time_loop.F90:58-92), replayed from a captured HIP graph. */
 int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
 
+/* Autotuner (not a reference routine): times every execution plan of this
+ * lattice -- the two-pass kernels and, for binary lattices, the one-pass
+ * z-march kernel over its tile shapes and z-chunks -- on trial_steps trial
+ * steps each, keeps the fastest and writes a text report (may be NULL).  The
+ * lattice state is unchanged. */
+int lb3d_gpu_autotune(lb3d_ctx *ctx, int32_t trial_steps, char *report, int32_t report_len);
+/* Choose the execution plan directly: fused = 0 (two-pass) or 1 (one-pass
+ * kernel with tile tx x ty and z-chunk lz; binary only). */
+int lb3d_gpu_set_plan(lb3d_ctx *ctx, int32_t fused, int32_t tx, int32_t ty, int32_t lz);
+
 /* Current step counter nt. */
 int lb3d_gpu_get_nt(lb3d_ctx *ctx, int32_t *nt);
 

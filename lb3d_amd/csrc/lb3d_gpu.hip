@@ -34,8 +34,16 @@ int fail(int code, const std::string &msg)
 enum StageState { IDLE = 0, AFTER_BC6 = 1, AFTER_ADV = 2, AFTER_BC8 = 3 };
 
 const char *const KNAMES[] = {"le_neighbours", "le_faces_pc", "moments",
-                              "le_faces_aux",  "dipole",      "collide"};
-enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_COUNT };
+                              "le_faces_aux",  "dipole",      "collide", "fused"};
+enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_FUS, K_COUNT };
+
+// How a timestep is executed: the two-pass sequence (moments, [dipole],
+// collide) or, for binary lattices, the one-pass z-march kernel with a tile
+// shape (tx, ty) and a z-chunk lz.
+struct Plan {
+    bool fused = false;
+    int tx = 64, ty = 4, lz = 0;
+};
 
 }  // namespace
 
@@ -53,6 +61,9 @@ struct lb3d_ctx {
     double *d_partials = nullptr;
     size_t f_bytes = 0, aux_bytes = 0, db_bytes = 0;
     int naux = 0;
+    Plan plan;
+    bool tuned = false;
+    bool aux_from_prev = false;  // aux is recomputable from f[1-cur] (fused binary steps)
 };
 
 static int check_params(const lb3d_params *p)
@@ -127,6 +138,13 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
         (e = hipMalloc((void **)&c->d_partials, (size_t)c->L->diag_blocks() * NDIAG * sizeof(double))) !=
             hipSuccess)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (p->ncomp == 2) {
+        // default one-pass plan; lb3d_gpu_autotune picks the measured best
+        c->plan.fused = true;
+        c->plan.tx = c->L->fused_supported(64, 4) ? 64 : 32;
+        c->plan.ty = 4;
+        c->plan.lz = p->nz < 64 ? p->nz : 64;
+    }
     c->host_le.shear_u = p->shear_u;
     c->host_le.shear_tmp = p->shear_u;
     c->host_le.shear_sum = p->shear_sum;
@@ -222,6 +240,7 @@ int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, i
     c->host_le.nt = nt0;
     c->state = IDLE;
     c->aux_valid = false;
+    c->aux_from_prev = false;
     return LB3D_OK;
 }
 
@@ -238,6 +257,7 @@ int lb3d_gpu_init_noise(lb3d_ctx *c, uint64_t seed, const double rho_mean[3], co
     c->host_le.nt = 0;
     c->state = IDLE;
     c->aux_valid = false;
+    c->aux_from_prev = false;
     return LB3D_OK;
 }
 
@@ -292,6 +312,16 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
         }
         return LB3D_OK;
     };
+    if (!c->aux_valid && c->aux_from_prev) {
+        // the one-pass plan keeps no moments: recompute those of the last
+        // advection from the previous post-collision populations f[1-cur]
+        c->b.cur = 1 - c->b.cur;
+        c->L->moments(c->b, c->stream);
+        c->b.cur = 1 - c->b.cur;
+        HIP_TRY(hipGetLastError());
+        c->aux_valid = true;
+        c->aux_from_prev = false;
+    }
     if (c->aux_valid && (rc = fetch(c->b.aux, (size_t)c->naux * ns, aux))) return rc;
     if (C == 3 && (rc = fetch(c->b.db, 3 * ns, db))) return rc;
     const size_t lx = nx + 2 * h, ly = ny + 2 * h, lz = nz + 2 * h;
@@ -421,10 +451,29 @@ int lb3d_gpu_advection(lb3d_ctx *c)
     g_err.clear();
     if (!c) return fail(LB3D_EINVAL, "NULL context");
     if (c->state != AFTER_BC6) return fail(LB3D_ESTATE, "advection must follow bc_apply(6)");
-    c->L->moments(c->b, c->stream);
-    HIP_TRY(hipGetLastError());
+    // the one-pass plan streams inside the collision kernel
+    if (!c->plan.fused) {
+        c->L->moments(c->b, c->stream);
+        HIP_TRY(hipGetLastError());
+    }
     c->state = AFTER_ADV;
     return LB3D_OK;
+}
+
+static void launch_collision(lb3d_ctx *c)
+{
+    if (c->plan.fused) {
+        c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->stream);
+    } else {
+        c->L->dipole(c->b, c->stream);
+        c->L->collide(c->b, c->stream);
+    }
+}
+
+static void mark_stepped(lb3d_ctx *c)
+{
+    c->aux_valid = !c->plan.fused;
+    c->aux_from_prev = c->plan.fused;
 }
 
 int lb3d_gpu_collision(lb3d_ctx *c)
@@ -432,12 +481,11 @@ int lb3d_gpu_collision(lb3d_ctx *c)
     g_err.clear();
     if (!c) return fail(LB3D_EINVAL, "NULL context");
     if (c->state != AFTER_BC8) return fail(LB3D_ESTATE, "collision must follow bc_apply(8)");
-    c->L->dipole(c->b, c->stream);
-    c->L->collide(c->b, c->stream);
+    launch_collision(c);
     HIP_TRY(hipGetLastError());
     c->b.cur = 1 - c->b.cur;
     c->state = IDLE;
-    c->aux_valid = true;
+    mark_stepped(c);
     return LB3D_OK;
 }
 
@@ -448,10 +496,13 @@ static void enqueue_step(lb3d_ctx *c)
         launch_le_neighbours(c, -1);
         c->L->le_faces(c->b, false, c->stream);
     }
-    c->L->moments(c->b, c->stream);
-    if (c->le) c->L->le_faces(c->b, true, c->stream);
-    c->L->dipole(c->b, c->stream);
-    c->L->collide(c->b, c->stream);
+    if (c->plan.fused) {
+        if (c->le) c->L->le_faces(c->b, true, c->stream);
+    } else {
+        c->L->moments(c->b, c->stream);
+        if (c->le) c->L->le_faces(c->b, true, c->stream);
+    }
+    launch_collision(c);
     c->b.cur = 1 - c->b.cur;
 }
 
@@ -477,6 +528,19 @@ static int ensure_graph(lb3d_ctx *c)
     return LB3D_OK;
 }
 
+static int sync_le_counter(lb3d_ctx *c)
+{
+    if (!c->le) return LB3D_OK;
+    // the device step counter follows the host's nt
+    int rc = pull_le(c);
+    if (rc) return rc;
+    if (c->host_le.nt != c->nt) {
+        c->host_le.nt = c->nt;
+        if ((rc = push_le(c))) return rc;
+    }
+    return LB3D_OK;
+}
+
 int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
 {
     g_err.clear();
@@ -484,18 +548,9 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     if (nsteps < 0) return fail(LB3D_EINVAL, "nsteps must be >= 0");
     if (c->state != IDLE) return fail(LB3D_ESTATE, "run_steps needs a completed timestep");
     if (nsteps == 0) return LB3D_OK;
-    if (c->le) {
-        // the device counter follows the host's nt
-        c->host_le.nt = c->nt;
-        int rc = pull_le(c);
-        if (rc) return rc;
-        if (c->host_le.nt != c->nt) {
-            c->host_le.nt = c->nt;
-            if ((rc = push_le(c))) return rc;
-        }
-    }
-    int rc = ensure_graph(c);
+    int rc = sync_le_counter(c);
     if (rc) return rc;
+    if ((rc = ensure_graph(c))) return rc;
     int left = nsteps;
     if (c->b.cur == 1) {
         enqueue_step(c);
@@ -505,7 +560,120 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     if (left == 1) enqueue_step(c);
     HIP_TRY(hipGetLastError());
     c->nt += nsteps;
-    c->aux_valid = true;
+    mark_stepped(c);
+    return LB3D_OK;
+}
+
+// Autotuner: times each candidate plan on the context's own lattice.  Trial
+// steps read f[cur] and write f[1-cur] without swapping, so the state is
+// unchanged (the LE scalars are not advanced; the ternary dipole field is
+// saved and restored around the trials).
+int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t report_len)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "autotune needs a completed timestep");
+    if (trial_steps < 1) trial_steps = 1;
+    std::vector<Plan> cands;
+    Plan two;
+    two.fused = false;
+    cands.push_back(two);
+    if (c->p.ncomp == 2) {
+        const int lzs[] = {16, 32, 64, 128, 256};
+        for (int k = 0; k < N_FUSED_SHAPES; k++)
+            for (int lz : lzs) {
+                if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1])) continue;
+                if (lz > c->p.nz && lz != 16) continue;
+                Plan q;
+                q.fused = true;
+                q.tx = FUSED_SHAPES[k][0];
+                q.ty = FUSED_SHAPES[k][1];
+                q.lz = lz < c->p.nz ? lz : c->p.nz;
+                cands.push_back(q);
+            }
+    }
+    void *db_save = nullptr;
+    if (c->p.ncomp == 3) {
+        HIP_TRY(hipMalloc(&db_save, c->db_bytes));
+        HIP_TRY(hipMemcpyAsync(db_save, c->b.db, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    Plan saved = c->plan;
+    double best = 1e300;
+    Plan bestp = saved;
+    std::string rep;
+    for (const Plan &q : cands) {
+        c->plan = q;
+        auto trial = [&]() {
+            if (c->le) c->L->le_faces(c->b, false, c->stream);
+            if (!q.fused) c->L->moments(c->b, c->stream);
+            if (c->le) c->L->le_faces(c->b, true, c->stream);
+            launch_collision(c);
+        };
+        trial();  // warm-up
+        HIP_TRY(hipEventRecord(e0, c->stream));
+        for (int t = 0; t < trial_steps; t++) trial();
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        double per = ms / trial_steps;
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "%s%s %dx%d lz=%d %.4f ms\n", q.fused ? "fused" : "two-pass", "",
+                      q.fused ? q.tx : 0, q.fused ? q.ty : 0, q.fused ? q.lz : 0, per);
+        rep += buf;
+        if (per < best) {
+            best = per;
+            bestp = q;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (db_save) {
+        HIP_TRY(hipMemcpyAsync(c->b.db, db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        (void)hipFree(db_save);
+    }
+    c->plan = bestp;
+    c->tuned = true;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    // aux (ternary two-pass) was overwritten by the trials with values of the same state
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %.4f ms\n", bestp.fused ? "fused" : "two-pass",
+                  bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0, best);
+    rep += buf;
+    if (report && report_len > 0) {
+        std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_t lz)
+{
+    g_err.clear();
+    if (!c) return fail(LB3D_EINVAL, "NULL context");
+    Plan q;
+    q.fused = fused != 0;
+    if (q.fused) {
+        if (c->p.ncomp != 2) return fail(LB3D_EINVAL, "the one-pass kernel is binary only");
+        bool ok = c->L->fused_supported(tx, ty);
+        if (!ok || lz < 1) return fail(LB3D_EINVAL, "no such tile shape");
+        q.tx = tx;
+        q.ty = ty;
+        q.lz = lz < c->p.nz ? lz : c->p.nz;
+    }
+    c->plan = q;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
     return LB3D_OK;
 }
 
@@ -597,13 +765,8 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
     if (!c || !names || !ms || !nkernels || !ms_per_step) return fail(LB3D_EINVAL, "NULL argument");
     if (nsteps < 1 || max_kernels < K_COUNT) return fail(LB3D_EINVAL, "nsteps >= 1 and room for 6 kernels needed");
     if (c->state != IDLE) return fail(LB3D_ESTATE, "profile_steps needs a completed timestep");
-    if (c->le) {
-        c->host_le.nt = c->nt;
-        int rc = pull_le(c);
-        if (rc) return rc;
-        c->host_le.nt = c->nt;
-        if ((rc = push_le(c))) return rc;
-    }
+    int rc0 = sync_le_counter(c);
+    if (rc0) return rc0;
     std::vector<hipEvent_t> ev((size_t)nsteps * K_COUNT * 2, nullptr);
     std::vector<int> used((size_t)nsteps * K_COUNT, 0);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
@@ -615,6 +778,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
     };
     HIP_TRY(hipEventRecord(t0, c->stream));
     for (int st = 0; st < nsteps; st++) {
+        auto mark = [&](int k) { used[(size_t)st * K_COUNT + k] = 1; };
         if (c->le) {
             HIP_TRY(rec(st, K_LEN, true));
             launch_le_neighbours(c, -1);
@@ -622,28 +786,38 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
             HIP_TRY(rec(st, K_LE6, true));
             c->L->le_faces(c->b, false, c->stream);
             HIP_TRY(rec(st, K_LE6, false));
-            used[(size_t)st * K_COUNT + K_LEN] = used[(size_t)st * K_COUNT + K_LE6] = 1;
+            mark(K_LEN);
+            mark(K_LE6);
         }
-        HIP_TRY(rec(st, K_MOM, true));
-        c->L->moments(c->b, c->stream);
-        HIP_TRY(rec(st, K_MOM, false));
-        used[(size_t)st * K_COUNT + K_MOM] = 1;
+        if (!c->plan.fused) {
+            HIP_TRY(rec(st, K_MOM, true));
+            c->L->moments(c->b, c->stream);
+            HIP_TRY(rec(st, K_MOM, false));
+            mark(K_MOM);
+        }
         if (c->le) {
             HIP_TRY(rec(st, K_LE8, true));
             c->L->le_faces(c->b, true, c->stream);
             HIP_TRY(rec(st, K_LE8, false));
-            used[(size_t)st * K_COUNT + K_LE8] = 1;
+            mark(K_LE8);
         }
-        if (c->p.ncomp == 3) {
-            HIP_TRY(rec(st, K_DIP, true));
-            c->L->dipole(c->b, c->stream);
-            HIP_TRY(rec(st, K_DIP, false));
-            used[(size_t)st * K_COUNT + K_DIP] = 1;
+        if (c->plan.fused) {
+            HIP_TRY(rec(st, K_FUS, true));
+            c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->stream);
+            HIP_TRY(rec(st, K_FUS, false));
+            mark(K_FUS);
+        } else {
+            if (c->p.ncomp == 3) {
+                HIP_TRY(rec(st, K_DIP, true));
+                c->L->dipole(c->b, c->stream);
+                HIP_TRY(rec(st, K_DIP, false));
+                mark(K_DIP);
+            }
+            HIP_TRY(rec(st, K_COL, true));
+            c->L->collide(c->b, c->stream);
+            HIP_TRY(rec(st, K_COL, false));
+            mark(K_COL);
         }
-        HIP_TRY(rec(st, K_COL, true));
-        c->L->collide(c->b, c->stream);
-        HIP_TRY(rec(st, K_COL, false));
-        used[(size_t)st * K_COUNT + K_COL] = 1;
         c->b.cur = 1 - c->b.cur;
     }
     HIP_TRY(hipEventRecord(t1, c->stream));
@@ -674,7 +848,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
     *nkernels = nk;
     *ms_per_step = all / nsteps;
     c->nt += nsteps;
-    c->aux_valid = true;
+    mark_stepped(c);
     return LB3D_OK;
 }
 

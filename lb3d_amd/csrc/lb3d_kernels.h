@@ -39,6 +39,25 @@ __device__ __forceinline__ uint32_t sidx(int x, int y, int z, int nx, int ny)
     return (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
 }
 
+// Loads and stores through a wave-uniform plane base plus a 32-bit byte offset
+// per lane: the plane bases live in SGPRs and each access needs one VGPR for
+// its offset (the global "saddr" addressing form) instead of a 64-bit address.
+template <typename R>
+__device__ __forceinline__ R ld_plane(const R *__restrict__ plane, uint32_t site)
+{
+    return *(const R *)((const char *)plane + site * (uint32_t)sizeof(R));
+}
+template <typename R>
+__device__ __forceinline__ void st_plane(R *__restrict__ plane, uint32_t site, R v)
+{
+    *(R *)((char *)plane + site * (uint32_t)sizeof(R)) = v;
+}
+template <typename R>
+__device__ __forceinline__ const R *plane_of(const R *__restrict__ base, uint32_t k, uint32_t n)
+{
+    return base + (size_t)k * (size_t)n;
+}
+
 // Population i of component c pulled into site (x,y,z): the value at
 // (x-c_i, y-c_i, z-c_i).  With Lees-Edwards a source outside 0..nx-1 in x
 // reads the x-halo plane face_pc.
@@ -51,13 +70,13 @@ __device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__
         if (xs < 0 || xs >= P.nx) {
             int fc = xs < 0 ? 0 : 1;
             uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
-            return face_pc[((uint32_t)(fc * C + c) * 19u + (uint32_t)i) * plane +
-                           (uint32_t)zs * (uint32_t)P.ny + (uint32_t)ys];
+            return ld_plane(plane_of(face_pc, (uint32_t)(fc * C + c) * 19u + (uint32_t)i, plane),
+                            (uint32_t)zs * (uint32_t)P.ny + (uint32_t)ys);
         }
     } else {
         xs = wrap1(xs, P.nx);
     }
-    return f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + sidx(xs, ys, zs, P.nx, P.ny)];
+    return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), sidx(xs, ys, zs, P.nx, P.ny));
 }
 
 // Value k of the aux field at the neighbour (x+dx, y+dy, z+dz); for LE the
@@ -71,13 +90,13 @@ __device__ __forceinline__ R aux_at(const R *__restrict__ aux, const R *__restri
         if (x < 0 || x >= P.nx) {
             int fc = x < 0 ? 0 : 1;
             uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
-            return face_aux[(uint32_t)(fc * 4 + k) * plane + (uint32_t)zs * (uint32_t)P.ny +
-                            (uint32_t)ys];
+            return ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + k), plane),
+                            (uint32_t)zs * (uint32_t)P.ny + (uint32_t)ys);
         }
     } else {
         x = wrap1(x, P.nx);
     }
-    return aux[(uint32_t)k * P.ns + sidx(x, ys, zs, P.nx, P.ny)];
+    return ld_plane(plane_of(aux, (uint32_t)k, P.ns), sidx(x, ys, zs, P.nx, P.ny));
 }
 
 // psi(rho) = 1 - exp(-rho) (get_psi, code/lb3d_force_shanchen.F90:334-353);
@@ -286,22 +305,14 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
 }
 
 // --------------------------------------------------------------------------
-// k_collide: pull, moments, Shan-Chen forces, body force, BGK relaxation
-// towards rho*boltz(u_tilde, du), code/lb3d_collision.F90:151-289 with
-// lb3d_get_sc_u_eq (:560-587) and lb3d_get_sc_du (:523-556).  Writes the
-// post-collision populations of the site into fout.
-template <typename R, bool Dev, int C, bool LE>
-__global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
-                                                 const R *__restrict__ face_pc,
-                                                 const R *__restrict__ aux,
-                                                 const R *__restrict__ face_aux,
-                                                 const R *__restrict__ db, R *__restrict__ fout)
+// Collision of one site from its streamed populations n and Shan-Chen forces
+// F: moments, body force, BGK relaxation towards rho*boltz(u_tilde, du),
+// code/lb3d_collision.F90:188-289 with lb3d_get_sc_u_eq (:560-587) and
+// lb3d_get_sc_du (:523-556).  Writes the post-collision populations of site s.
+template <typename R, bool Dev, int C>
+__device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
+                                         int x, int y, int z, uint32_t s, R *__restrict__ fout)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.ns) return;
-    int x, y, z;
-    site_xyz(s, P.nx, P.ny, x, y, z);
-    R n[C][19];
     R rho[C], mom[C][3];
 #pragma unroll
     for (int c = 0; c < C; c++) {
@@ -312,8 +323,7 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
     for (int i = 0; i < 19; i++) {
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            R v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
-            n[c][i] = v;
+            R v = n[c][i];
             rho[c] = rho[c] + v * R(GD[i]);
             R ng = v * R(GD[i]);
             if (CX[i] != 0) mom[c][0] = mom[c][0] + ng * R(CX[i]);
@@ -321,8 +331,6 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
             if (CZ[i] != 0) mom[c][2] = mom[c][2] + ng * R(CZ[i]);
         }
     }
-    R F[C][3];
-    sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
     R rhof[C];  // full densities
 #pragma unroll
     for (int c = 0; c < C; c++) rhof[c] = Dev ? P.rho0[c] + rho[c] : rho[c];
@@ -359,7 +367,6 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
         R du1 = (tp * F[c][1]) / P.amass[c];
         R du2 = (tp * F[c][2]) / P.amass[c];
         R b[19];
-        uint32_t base = (uint32_t)c * 19u * P.ns + s;
         if constexpr (Dev) {
             boltz_dev(P.bdist, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
@@ -367,16 +374,196 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
                 // deviation of rho*b_i from w_i*rho0: w_i*drho + rho*(b_i - w_i)
                 R wi = i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0);
                 R deq = wi * rho[c] + rhof[c] * b[i];
-                fout[base + (uint32_t)i * P.ns] = n[c][i] - P.omega[c] * (n[c][i] - deq);
+                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s,
+                         n[c][i] - P.omega[c] * (n[c][i] - deq));
             }
         } else {
             boltz_dist<R>(P.bdist, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];
-                fout[base + (uint32_t)i * P.ns] = n[c][i] - P.omega[c] * (n[c][i] - neq);
+                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s,
+                         n[c][i] - P.omega[c] * (n[c][i] - neq));
             }
         }
+    }
+}
+
+// --------------------------------------------------------------------------
+// k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
+// of the two-pass step (k_moments then k_collide).
+template <typename R, bool Dev, int C, bool LE>
+__global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
+                                                 const R *__restrict__ face_pc,
+                                                 const R *__restrict__ aux,
+                                                 const R *__restrict__ face_aux,
+                                                 const R *__restrict__ db, R *__restrict__ fout)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R n[C][19];
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+#pragma unroll
+        for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+    R F[C][3];
+    sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+    bgk_site<R, Dev, C>(P, n, F, x, y, z, s, fout);
+}
+
+// --------------------------------------------------------------------------
+// k_fused2: the whole binary timestep in one pass.  A workgroup owns a TX x TY
+// column tile and marches along z through [z0, z0+lz).  Each thread pulls the
+// streamed populations of its site one plane ahead (plane k+1) and keeps them
+// in registers while it collides plane k; the densities psi of plane k+1 for
+// the tile and its one-site halo go into an LDS ring of three planes, the
+// halo sites by a sum-only pull.  Nothing but the populations is read from or
+// written to memory (Lees-Edwards halo columns read face_pc / face_aux).
+// Each population is read once by its owner; halo pulls re-read neighbours'
+// data, which adjacent tiles on the same XCD fetch at the same time.
+template <typename R, bool Dev, bool LE>
+__device__ __forceinline__ void psi_at(const Phys<R> &P, const R *__restrict__ f,
+                                       const R *__restrict__ face_pc,
+                                       const R *__restrict__ face_aux, int x, int y, int z,
+                                       R &p0, R &p1)
+{
+    if (LE && (x < 0 || x >= P.nx)) {
+        uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+        int fc = x < 0 ? 0 : 1;
+        uint32_t q = (uint32_t)wrap1(z, P.nz) * (uint32_t)P.ny + (uint32_t)wrap1(y, P.ny);
+        p0 = ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + 0), plane), q);
+        p1 = ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + 1), plane), q);
+        return;
+    }
+    int xs = wrap1(x, P.nx), ys = wrap1(y, P.ny), zs = wrap1(z, P.nz);
+    R r0 = R(0), r1 = R(0);
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        r0 = r0 + pull<R, 2, LE>(f, face_pc, P, 0, i, xs, ys, zs) * R(GD[i]);
+        r1 = r1 + pull<R, 2, LE>(f, face_pc, P, 1, i, xs, ys, zs) * R(GD[i]);
+    }
+    p0 = psi_of<Dev>(r0, P.emr0[0]);
+    p1 = psi_of<Dev>(r1, P.emr0[1]);
+}
+
+template <typename R, bool Dev, bool LE, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restrict__ f,
+                                                    const R *__restrict__ face_pc,
+                                                    const R *__restrict__ face_aux,
+                                                    R *__restrict__ fout, int lz)
+{
+    constexpr int T = TX * TY, LX = TX + 2, LY = TY + 2, LP = LX * LY;
+    constexpr int NH = 2 * LX + 2 * TY;  // halo positions per plane
+    __shared__ R ring[3][2][LP];         // psi of planes k-1, k, k+1 (tile + halo)
+    __shared__ R stage[38][T];           // populations of plane k+1, one column per thread
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * lz;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool own = x < P.nx && y < P.ny;
+    const int zend = min(z0 + lz, P.nz);
+    if (z0 >= zend) return;  // uniform over the block
+    // LDS positions this thread fills besides its own: up to two halo slots
+    int hq[2], hgx[2], hgy[2];
+    bool hv[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        int h = tid + r * T;
+        int ax = 0, ay = 0;
+        if (h < LX) { ax = h - 1; ay = -1; }
+        else if (h < 2 * LX) { ax = h - LX - 1; ay = TY; }
+        else if (h < 2 * LX + TY) { ax = -1; ay = h - 2 * LX; }
+        else { ax = TX; ay = h - 2 * LX - TY; }
+        hq[r] = (ay + 1) * LX + (ax + 1);
+        hgx[r] = x0 + ax;
+        hgy[r] = y0 + ay;
+        // needed: at most one site beyond the lattice in x and y
+        hv[r] = h < NH && x0 + ax <= P.nx && y0 + ay <= P.ny;
+    }
+    const bool mid_needed = !own && x <= P.nx && y <= P.ny;  // an interior slot acting as halo
+    const int me = (ly + 1) * LX + (lx + 1);
+
+    // psi of plane zp for the tile and halo into ring slot; the owner's
+    // streamed populations of that plane go to its stage column
+    auto fill = [&](int zp, int slot) __attribute__((always_inline)) {
+        if (own) {
+            int zs = wrap1(zp, P.nz);
+            R r0 = R(0), r1 = R(0);
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                R v0 = pull<R, 2, LE>(f, face_pc, P, 0, i, x, y, zs);
+                R v1 = pull<R, 2, LE>(f, face_pc, P, 1, i, x, y, zs);
+                stage[i][tid] = v0;
+                stage[19 + i][tid] = v1;
+                r0 = r0 + v0 * R(GD[i]);
+                r1 = r1 + v1 * R(GD[i]);
+            }
+            ring[slot][0][me] = psi_of<Dev>(r0, P.emr0[0]);
+            ring[slot][1][me] = psi_of<Dev>(r1, P.emr0[1]);
+        } else if (mid_needed) {
+            R p0, p1;
+            psi_at<R, Dev, LE>(P, f, face_pc, face_aux, x, y, zp, p0, p1);
+            ring[slot][0][me] = p0;
+            ring[slot][1][me] = p1;
+        }
+#pragma unroll
+        for (int r = 0; r < (NH > T ? 2 : 1); r++)
+            if (hv[r]) {
+                R p0, p1;
+                psi_at<R, Dev, LE>(P, f, face_pc, face_aux, hgx[r], hgy[r], zp, p0, p1);
+                ring[slot][0][hq[r]] = p0;
+                ring[slot][1][hq[r]] = p1;
+            }
+    };
+    // prologue: planes z0-1 and z0
+    fill(z0 - 1, 0);
+    fill(z0, 1);
+    int sm = 0, s0 = 1, sp = 2;  // ring slots of planes k-1, k, k+1
+    for (int k = z0; k < zend; k++) {
+        // the populations of plane k from the stage (own column only)
+        R cur[2][19];
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            cur[0][i] = stage[i][tid];
+            cur[1][i] = stage[19 + i][tid];
+        }
+        fill(k + 1, sp);
+        __syncthreads();
+        if (own) {
+            R frb0 = R(0), frb1 = R(0), frb2 = R(0), fbr0 = R(0), fbr1 = R(0), fbr2 = R(0);
+#pragma unroll
+            for (int i = 0; i < 18; i++) {
+                int sl = CZ[i] == 0 ? s0 : (CZ[i] > 0 ? sp : sm);
+                int q = me + CY[i] * LX + CX[i];
+                R pr = ring[sl][0][q];
+                R pb = ring[sl][1][q];
+                R green_r = pb * R(GD[i]);
+                R green_b = pr * R(GD[i]);
+                if (CX[i] != 0) { frb0 = frb0 + green_r * R(CX[i]); fbr0 = fbr0 + green_b * R(CX[i]); }
+                if (CY[i] != 0) { frb1 = frb1 + green_r * R(CY[i]); fbr1 = fbr1 + green_b * R(CY[i]); }
+                if (CZ[i] != 0) { frb2 = frb2 + green_r * R(CZ[i]); fbr2 = fbr2 + green_b * R(CZ[i]); }
+            }
+            R psr = ring[s0][0][me], psb = ring[s0][1][me];
+            if constexpr (Dev) {
+                psr = P.psi0[0] + psr;
+                psb = P.psi0[1] + psb;
+            }
+            R F[2][3];
+            F[1][0] = (-(P.g_br * fbr0)) * psb;
+            F[1][1] = (-(P.g_br * fbr1)) * psb;
+            F[1][2] = (-(P.g_br * fbr2)) * psb;
+            F[0][0] = (-(P.g_br * frb0)) * psr;
+            F[0][1] = (-(P.g_br * frb1)) * psr;
+            F[0][2] = (-(P.g_br * frb2)) * psr;
+            bgk_site<R, Dev, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+        }
+        __syncthreads();
+        int t = sm;
+        sm = s0;
+        s0 = sp;
+        sp = t;
     }
 }
 

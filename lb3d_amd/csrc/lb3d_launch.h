@@ -199,6 +199,54 @@ class LauncherT : public Launcher {
         else
             k_le_faces<R, Dev, false><<<nb, 128, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
     }
+    // LDS of k_fused2: the population stage plus the psi ring
+    template <int TX, int TY>
+    static constexpr bool fits()
+    {
+        return (38 * TX * TY + 6 * (TX + 2) * (TY + 2)) * sizeof(R) <= 96 * 1024;
+    }
+    template <int TX, int TY>
+    bool fused_t(const DevBufs &b, int lz, hipStream_t s, bool launch)
+    {
+        if constexpr (!fits<TX, TY>()) {
+            return false;
+        } else {
+            if (launch) fused_go<TX, TY>(b, lz, s);
+            return true;
+        }
+    }
+    template <int TX, int TY>
+    void fused_go(const DevBufs &b, int lz, hipStream_t s)
+    {
+        dim3 grid((P.nx + TX - 1) / TX, (P.ny + TY - 1) / TY, (P.nz + lz - 1) / lz);
+        const R *f = (const R *)b.f[b.cur];
+        R *fo = (R *)b.f[1 - b.cur];
+        if (le)
+            k_fused2<R, Dev, true, TX, TY><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
+                                                                     (const R *)b.face_aux, fo, lz);
+        else
+            k_fused2<R, Dev, false, TX, TY><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz);
+    }
+    bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, hipStream_t s, bool launch)
+    {
+        if (C != 2 || lz < 1) return false;
+        if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, s, launch);
+        if (tx == 32 && ty == 8) return fused_t<32, 8>(b, lz, s, launch);
+        if (tx == 64 && ty == 8) return fused_t<64, 8>(b, lz, s, launch);
+        if (tx == 32 && ty == 16) return fused_t<32, 16>(b, lz, s, launch);
+        if (tx == 128 && ty == 4) return fused_t<128, 4>(b, lz, s, launch);
+        if (tx == 32 && ty == 4) return fused_t<32, 4>(b, lz, s, launch);
+        return false;
+    }
+    bool fused(const DevBufs &b, int tx, int ty, int lz, hipStream_t s) override
+    {
+        return fused_dispatch(b, tx, ty, lz, s, true);
+    }
+    bool fused_supported(int tx, int ty) override
+    {
+        DevBufs none;
+        return fused_dispatch(none, tx, ty, 1, nullptr, false);
+    }
     void init_noise(const DevBufs &b, uint64_t seed, const double m[3], const double a[3], int dip,
                     hipStream_t s) override
     {

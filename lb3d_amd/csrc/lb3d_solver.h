@@ -26,6 +26,10 @@ class Launcher {
     virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
     virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
     virtual void le_faces(const DevBufs &b, bool post, hipStream_t s) = 0;
+    // the whole binary step in one kernel (f[cur] -> f[1-cur]); false if the
+    // tile shape has no instantiation or the form is not binary
+    virtual bool fused(const DevBufs &b, int tx, int ty, int lz, hipStream_t s) = 0;
+    virtual bool fused_supported(int tx, int ty) = 0;
     virtual void init_noise(const DevBufs &b, uint64_t seed, const double rho_mean[3],
                             const double amp[3], int dipoles, hipStream_t s) = 0;
     // per-block FP64 partial sums for the sanity reductions (NDIAG per block)
@@ -33,6 +37,10 @@ class Launcher {
     virtual unsigned diag_blocks() const = 0;
     virtual size_t real_bytes() const = 0;
 };
+
+// Tile shapes of k_fused2 that are instantiated (TX, TY).
+constexpr int FUSED_SHAPES[][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {128, 4}, {32, 4}};
+constexpr int N_FUSED_SHAPES = sizeof(FUSED_SHAPES) / sizeof(FUSED_SHAPES[0]);
 
 constexpr int NDIAG = 9;  // mass[3], momentum[3], nan sites, negative sites, fast sites
 constexpr unsigned BLOCK = 256;

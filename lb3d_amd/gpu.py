@@ -34,7 +34,7 @@ EXPORTS = [
     "lb3d_gpu_bc_init", "lb3d_gpu_force_apply", "lb3d_gpu_bc_apply", "lb3d_gpu_advection",
     "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
     "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
-    "lb3d_gpu_get_density", "lb3d_gpu_synchronize",
+    "lb3d_gpu_get_density", "lb3d_gpu_synchronize", "lb3d_gpu_autotune", "lb3d_gpu_set_plan",
 ]
 
 
@@ -91,6 +91,8 @@ def lib():
         L.lb3d_gpu_get_populations.argtypes = [vp, dp]
         L.lb3d_gpu_get_density.argtypes = [vp, dp]
         L.lb3d_gpu_synchronize.argtypes = [vp]
+        L.lb3d_gpu_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
+        L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip]
         L.lb3d_gpu_profile_steps.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_char_p), dp,
                                              ctypes.POINTER(ip), dp]
         _lib = L
@@ -183,6 +185,16 @@ class Lattice:
         check(lib().lb3d_gpu_get_density(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               "get_density")
         return out
+
+    def autotune(self, trial_steps=3):
+        """Time every execution plan on this lattice and keep the fastest;
+        returns the tuner's report."""
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().lb3d_gpu_autotune(self._h, int(trial_steps), buf, len(buf)), "autotune")
+        return buf.value.decode()
+
+    def set_plan(self, fused, tx=64, ty=4, lz=64):
+        check(lib().lb3d_gpu_set_plan(self._h, int(fused), int(tx), int(ty), int(lz)), "set_plan")
 
     def synchronize(self):
         check(lib().lb3d_gpu_synchronize(self._h), "synchronize")

@@ -1,0 +1,82 @@
+"""Execution plans: the one-pass z-march kernel against the two-pass kernels
+and the oracle, over tile shapes, z-chunks and lattice sizes that do not
+divide into tiles; the autotuner leaves the lattice state unchanged."""
+import numpy as np
+import pytest
+
+from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64, LB3DError, Lattice
+from tests.test_gpu_parity import assert_pops_close, pair
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 4), (32, 8), (64, 8), (32, 16), (128, 4), (32, 4)]
+
+
+def fused_shapes(precision):
+    g = Lattice(8, 8, 8, ncomp=2, precision=precision)
+    ok = []
+    for tx, ty in SHAPES:
+        try:
+            g.set_plan(1, tx, ty, 4)
+            ok.append((tx, ty))
+        except LB3DError:
+            pass
+    return ok
+
+
+@pytest.mark.parametrize("precision", [LB3D_FP64, LB3D_FP32_DEV])
+@pytest.mark.parametrize("dims,le", [((20, 13, 11), False), ((7, 5, 3), False), ((70, 9, 6), True),
+                                      ((16, 12, 20), True)])
+def test_fused_equals_two_pass_bitwise(precision, dims, le):
+    kw = dict(ncomp=2, precision=precision, g_br=0.09)
+    if le:
+        kw.update(boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.04)
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0)
+    ref.time_loop(9)
+    want = ref.populations()
+    for tx, ty in fused_shapes(precision):
+        for lz in (1, 4, dims[2]):
+            _, g = pair(*dims, **kw)
+            g.set_plan(1, tx, ty, lz)
+            g.time_loop(9)
+            got = g.populations()
+            assert np.array_equal(got, want), f"{dims} le={le} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
+
+
+def test_fused_fp64_matches_oracle_32cubed_100_steps():
+    o, g = pair(32, 32, 32, ncomp=2, g_br=0.08)
+    g.set_plan(1, 32, 4, 16)
+    o.run(100)
+    g.time_loop(100)
+    assert_pops_close(o, g, what="fused binary 32^3 x100")
+
+
+def test_fused_staged_surface_and_download_moments():
+    """With the one-pass plan, advection() only advances the stage; the
+    downloaded rho are still those of the last advection."""
+    o, g = pair(12, 10, 8, ncomp=2)
+    g.set_plan(1, 32, 4, 8)
+    for _ in range(3):
+        g.time_step()
+    o.run(3)
+    assert_pops_close(o, g, what="fused staged x3")
+    N = g.download_sites()
+    np.testing.assert_allclose(N[1:-1, 1:-1, 1:-1, 19], o.interior()[..., 19], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(N[1:-1, 1:-1, 1:-1, 43], o.interior()[..., 43], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_autotune_keeps_state_and_picks_a_plan(ncomp):
+    kw = dict(ncomp=ncomp, amp=(0.01, 0.01, 0.01))
+    o, g = pair(24, 16, 12, **kw)
+    o.run(2)
+    g.time_loop(2)
+    before = g.populations()
+    rep = g.autotune(trial_steps=2)
+    print(rep)
+    assert "chosen:" in rep
+    np.testing.assert_array_equal(g.populations(), before)
+    o.run(5)
+    g.time_loop(5)
+    assert_pops_close(o, g, what=f"after autotune ncomp={ncomp}")
