@@ -448,7 +448,7 @@ __device__ __forceinline__ void psi_at(const Phys<R> &P, const R *__restrict__ f
     p1 = psi_of<Dev>(r1, P.emr0[1]);
 }
 
-template <typename R, bool Dev, bool LE, int TX, int TY>
+template <typename R, bool Dev, bool LE, int TX, int TY, bool Stage>
 __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restrict__ f,
                                                     const R *__restrict__ face_pc,
                                                     const R *__restrict__ face_aux,
@@ -457,7 +457,9 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     constexpr int T = TX * TY, LX = TX + 2, LY = TY + 2, LP = LX * LY;
     constexpr int NH = 2 * LX + 2 * TY;  // halo positions per plane
     __shared__ R ring[3][2][LP];         // psi of planes k-1, k, k+1 (tile + halo)
-    __shared__ R stage[38][T];           // populations of plane k+1, one column per thread
+    // populations of plane k+1: in LDS, one column per thread (Stage), or in registers
+    __shared__ R stage[Stage ? 38 : 1][Stage ? T : 1];
+    R nxt[2][19];
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = tid / TX;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * lz;
@@ -495,8 +497,13 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             for (int i = 0; i < 19; i++) {
                 R v0 = pull<R, 2, LE>(f, face_pc, P, 0, i, x, y, zs);
                 R v1 = pull<R, 2, LE>(f, face_pc, P, 1, i, x, y, zs);
-                stage[i][tid] = v0;
-                stage[19 + i][tid] = v1;
+                if constexpr (Stage) {
+                    stage[i][tid] = v0;
+                    stage[19 + i][tid] = v1;
+                } else {
+                    nxt[0][i] = v0;
+                    nxt[1][i] = v1;
+                }
                 r0 = r0 + v0 * R(GD[i]);
                 r1 = r1 + v1 * R(GD[i]);
             }
@@ -526,8 +533,13 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
         R cur[2][19];
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            cur[0][i] = stage[i][tid];
-            cur[1][i] = stage[19 + i][tid];
+            if constexpr (Stage) {
+                cur[0][i] = stage[i][tid];
+                cur[1][i] = stage[19 + i][tid];
+            } else {
+                cur[0][i] = nxt[0][i];
+                cur[1][i] = nxt[1][i];
+            }
         }
         fill(k + 1, sp);
         __syncthreads();

@@ -206,46 +206,50 @@ class LauncherT : public Launcher {
         return (38 * TX * TY + 6 * (TX + 2) * (TY + 2)) * sizeof(R) <= 96 * 1024;
     }
     template <int TX, int TY>
-    bool fused_t(const DevBufs &b, int lz, hipStream_t s, bool launch)
+    bool fused_t(const DevBufs &b, int lz, bool stage, hipStream_t s, bool launch)
     {
-        if constexpr (!fits<TX, TY>()) {
-            return false;
-        } else {
-            if (launch) fused_go<TX, TY>(b, lz, s);
-            return true;
+        if (stage) {
+            if constexpr (!fits<TX, TY>()) {
+                return false;
+            } else {
+                if (launch) fused_go<TX, TY, true>(b, lz, s);
+                return true;
+            }
         }
+        if (launch) fused_go<TX, TY, false>(b, lz, s);
+        return true;
     }
-    template <int TX, int TY>
+    template <int TX, int TY, bool ST>
     void fused_go(const DevBufs &b, int lz, hipStream_t s)
     {
         dim3 grid((P.nx + TX - 1) / TX, (P.ny + TY - 1) / TY, (P.nz + lz - 1) / lz);
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (le)
-            k_fused2<R, Dev, true, TX, TY><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
-                                                                     (const R *)b.face_aux, fo, lz);
+            k_fused2<R, Dev, true, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
+                                                                         (const R *)b.face_aux, fo, lz);
         else
-            k_fused2<R, Dev, false, TX, TY><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz);
+            k_fused2<R, Dev, false, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz);
     }
-    bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, hipStream_t s, bool launch)
+    bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, bool st, hipStream_t s, bool launch)
     {
         if (C != 2 || lz < 1) return false;
-        if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, s, launch);
-        if (tx == 32 && ty == 8) return fused_t<32, 8>(b, lz, s, launch);
-        if (tx == 64 && ty == 8) return fused_t<64, 8>(b, lz, s, launch);
-        if (tx == 32 && ty == 16) return fused_t<32, 16>(b, lz, s, launch);
-        if (tx == 128 && ty == 4) return fused_t<128, 4>(b, lz, s, launch);
-        if (tx == 32 && ty == 4) return fused_t<32, 4>(b, lz, s, launch);
+        if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, st, s, launch);
+        if (tx == 32 && ty == 8) return fused_t<32, 8>(b, lz, st, s, launch);
+        if (tx == 64 && ty == 8) return fused_t<64, 8>(b, lz, st, s, launch);
+        if (tx == 32 && ty == 16) return fused_t<32, 16>(b, lz, st, s, launch);
+        if (tx == 128 && ty == 4) return fused_t<128, 4>(b, lz, st, s, launch);
+        if (tx == 32 && ty == 4) return fused_t<32, 4>(b, lz, st, s, launch);
         return false;
     }
-    bool fused(const DevBufs &b, int tx, int ty, int lz, hipStream_t s) override
+    bool fused(const DevBufs &b, int tx, int ty, int lz, bool stage, hipStream_t s) override
     {
-        return fused_dispatch(b, tx, ty, lz, s, true);
+        return fused_dispatch(b, tx, ty, lz, stage, s, true);
     }
-    bool fused_supported(int tx, int ty) override
+    bool fused_supported(int tx, int ty, bool stage) override
     {
         DevBufs none;
-        return fused_dispatch(none, tx, ty, 1, nullptr, false);
+        return fused_dispatch(none, tx, ty, 1, stage, nullptr, false);
     }
     void init_noise(const DevBufs &b, uint64_t seed, const double m[3], const double a[3], int dip,
                     hipStream_t s) override

@@ -39,7 +39,14 @@ def main(ref):
             d = np.fromfile(os.path.join(base, "reference", f"{o}_{name}.reference-t00000010.xdr"),
                             dtype=">f4").astype("<f4")
             out[f"{name}/{o}"] = d.reshape(16, 16, 16, 3) if o in ("dir", "arr") else d.reshape(16, 16, 16)
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_goldens.npz")
+    here = os.path.dirname(os.path.abspath(__file__))
+    # the namelist input files of the same tests (data, read by the Fortran host)
+    os.makedirs(os.path.join(here, "inputs"), exist_ok=True)
+    for name in TESTS:
+        src = os.path.join(ref, "test", name, f"{name}.in")
+        with open(src) as fi, open(os.path.join(here, "inputs", f"{name}.in"), "w") as fo:
+            fo.write(fi.read())
+    dst = os.path.join(here, "reference_goldens.npz")
     np.savez_compressed(dst, **out)
     print("wrote", dst, os.path.getsize(dst), "bytes")
 
