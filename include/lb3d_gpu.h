@@ -135,8 +135,9 @@ int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
  * steps each, keeps the fastest and writes a text report (may be NULL).  The
  * lattice state is unchanged. */
 int lb3d_gpu_autotune(lb3d_ctx *ctx, int32_t trial_steps, char *report, int32_t report_len);
-/* Choose the execution plan directly: fused = 0 (two-pass) or 1 (one-pass
- * kernel with tile tx x ty and z-chunk lz; binary only). */
+/* Choose the execution plan directly: fused = 0 (two-pass), 1 (one-pass
+ * kernel with tile tx x ty and z-chunk lz, look-ahead plane staged in LDS) or
+ * 2 (the same with the look-ahead plane in registers); 1 and 2 binary only. */
 int lb3d_gpu_set_plan(lb3d_ctx *ctx, int32_t fused, int32_t tx, int32_t ty, int32_t lz);
 
 /* Current step counter nt. */

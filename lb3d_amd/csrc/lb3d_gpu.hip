@@ -43,6 +43,7 @@ enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_FUS, K_COUNT };
 struct Plan {
     bool fused = false;
     int tx = 64, ty = 4, lz = 0;
+    bool stage = true;  // look-ahead populations in LDS (true) or registers
 };
 
 }  // namespace
@@ -141,7 +142,7 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
     if (p->ncomp == 2) {
         // default one-pass plan; lb3d_gpu_autotune picks the measured best
         c->plan.fused = true;
-        c->plan.tx = c->L->fused_supported(64, 4) ? 64 : 32;
+        c->plan.tx = c->L->fused_supported(64, 4, true) ? 64 : 32;
         c->plan.ty = 4;
         c->plan.lz = p->nz < 64 ? p->nz : 64;
     }
@@ -463,7 +464,7 @@ int lb3d_gpu_advection(lb3d_ctx *c)
 static void launch_collision(lb3d_ctx *c)
 {
     if (c->plan.fused) {
-        c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->stream);
+        c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream);
     } else {
         c->L->dipole(c->b, c->stream);
         c->L->collide(c->b, c->stream);
@@ -581,16 +582,18 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     if (c->p.ncomp == 2) {
         const int lzs[] = {16, 32, 64, 128, 256};
         for (int k = 0; k < N_FUSED_SHAPES; k++)
-            for (int lz : lzs) {
-                if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1])) continue;
-                if (lz > c->p.nz && lz != 16) continue;
-                Plan q;
-                q.fused = true;
-                q.tx = FUSED_SHAPES[k][0];
-                q.ty = FUSED_SHAPES[k][1];
-                q.lz = lz < c->p.nz ? lz : c->p.nz;
-                cands.push_back(q);
-            }
+            for (int st = 0; st < 2; st++)
+                for (int lz : lzs) {
+                    if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1], st == 0)) continue;
+                    if (lz > c->p.nz && lz != 16) continue;
+                    Plan q;
+                    q.fused = true;
+                    q.tx = FUSED_SHAPES[k][0];
+                    q.ty = FUSED_SHAPES[k][1];
+                    q.lz = lz < c->p.nz ? lz : c->p.nz;
+                    q.stage = st == 0;
+                    cands.push_back(q);
+                }
     }
     void *db_save = nullptr;
     if (c->p.ncomp == 3) {
@@ -622,8 +625,9 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         double per = ms / trial_steps;
         char buf[96];
-        std::snprintf(buf, sizeof buf, "%s%s %dx%d lz=%d %.4f ms\n", q.fused ? "fused" : "two-pass", "",
-                      q.fused ? q.tx : 0, q.fused ? q.ty : 0, q.fused ? q.lz : 0, per);
+        std::snprintf(buf, sizeof buf, "%s %dx%d lz=%d %s %.4f ms\n", q.fused ? "fused" : "two-pass",
+                      q.fused ? q.tx : 0, q.fused ? q.ty : 0, q.fused ? q.lz : 0,
+                      q.fused ? (q.stage ? "lds" : "regs") : "-", per);
         rep += buf;
         if (per < best) {
             best = per;
@@ -645,8 +649,9 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     }
     // aux (ternary two-pass) was overwritten by the trials with values of the same state
     char buf[96];
-    std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %.4f ms\n", bestp.fused ? "fused" : "two-pass",
-                  bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0, best);
+    std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %s %.4f ms\n", bestp.fused ? "fused" : "two-pass",
+                  bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0,
+                  bestp.fused ? (bestp.stage ? "lds" : "regs") : "-", best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
@@ -661,9 +666,10 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     if (!c) return fail(LB3D_EINVAL, "NULL context");
     Plan q;
     q.fused = fused != 0;
+    q.stage = fused != 2;  // fused = 2: look-ahead in registers
     if (q.fused) {
         if (c->p.ncomp != 2) return fail(LB3D_EINVAL, "the one-pass kernel is binary only");
-        bool ok = c->L->fused_supported(tx, ty);
+        bool ok = c->L->fused_supported(tx, ty, q.stage);
         if (!ok || lz < 1) return fail(LB3D_EINVAL, "no such tile shape");
         q.tx = tx;
         q.ty = ty;
@@ -803,7 +809,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
         }
         if (c->plan.fused) {
             HIP_TRY(rec(st, K_FUS, true));
-            c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->stream);
+            c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream);
             HIP_TRY(rec(st, K_FUS, false));
             mark(K_FUS);
         } else {

@@ -117,12 +117,11 @@ __device__ __forceinline__ R psi_of(R rho, R emr0)
 // da = sum_i g_i n_s,i(x-c_i) db(x-c_i) (:130-148).  The streamed
 // populations themselves are not stored: k_collide pulls them again.
 template <typename R, bool Dev, int C, bool LE>
-__global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
-                                                 const R *__restrict__ face_pc,
-                                                 const R *__restrict__ db, R *__restrict__ aux)
+__device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
+                                             const R *__restrict__ face_pc,
+                                             const R *__restrict__ db, R *__restrict__ aux,
+                                             uint32_t s)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.ns) return;
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R rho[C];
@@ -142,39 +141,46 @@ __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict_
                     int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny),
                         zs = wrap1(z - CZ[i], P.nz);
                     uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
-                    da0 = da0 + ng * db[q];
-                    da1 = da1 + ng * db[P.ns + q];
-                    da2 = da2 + ng * db[2u * P.ns + q];
+                    da0 = da0 + ng * ld_plane(db, q);
+                    da1 = da1 + ng * ld_plane(plane_of(db, 1u, P.ns), q);
+                    da2 = da2 + ng * ld_plane(plane_of(db, 2u, P.ns), q);
                 }
             }
         }
     }
-    constexpr int NA = C == 3 ? NAUX3 : NAUX2;
-    (void)NA;
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        aux[(uint32_t)c * P.ns + s] = psi_of<Dev>(rho[c], P.emr0[c]);
-        aux[(uint32_t)(C + c) * P.ns + s] = rho[c];
+        st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
+        st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
     }
     if constexpr (C == 3) {
-        aux[6u * P.ns + s] = da0;
-        aux[7u * P.ns + s] = da1;
-        aux[8u * P.ns + s] = da2;
+        st_plane(aux + (size_t)6 * P.ns, s, da0);
+        st_plane(aux + (size_t)7 * P.ns, s, da1);
+        st_plane(aux + (size_t)8 * P.ns, s, da2);
     }
+}
+
+template <typename R, bool Dev, int C, bool LE>
+__global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
+                                                 const R *__restrict__ face_pc,
+                                                 const R *__restrict__ db, R *__restrict__ aux)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    moments_site<R, Dev, C, LE>(P, f, face_pc, db, aux, s);
 }
 
 // --------------------------------------------------------------------------
 // k_dipole: dipole relaxation, code/lb3d_collision.F90:296-403 (ternary).
 template <typename R, bool Dev>
-__global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__ aux,
-                                                R *__restrict__ db)
+__device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restrict__ aux,
+                                            R *__restrict__ db, uint32_t s)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.ns) return;
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
-    R hs0 = aux[6u * P.ns + s], hs1 = aux[7u * P.ns + s], hs2 = aux[8u * P.ns + s];
-    R rho_s = aux[5u * P.ns + s];
+    R hs0 = ld_plane(plane_of(aux, 6u, P.ns), s), hs1 = ld_plane(plane_of(aux, 7u, P.ns), s),
+      hs2 = ld_plane(plane_of(aux, 8u, P.ns), s);
+    R rho_s = ld_plane(plane_of(aux, 5u, P.ns), s);
     if constexpr (Dev) rho_s = P.rho0[2] + rho_s;
     R sN = rho_s > R(CLAMP_SURF) ? rho_s : R(CLAMP_SURF);
     R sd0 = hs0 / sN, sd1 = hs1 / sN, sd2 = hs2 / sN;
@@ -211,9 +217,18 @@ __global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__
     e0 = e0 * P.d_0;
     e1 = e1 * P.d_0;
     e2 = e2 * P.d_0;
-    db[s] = P.omega_d * e0 + P.one_m_omega_d * sd0;
-    db[P.ns + s] = P.omega_d * e1 + P.one_m_omega_d * sd1;
-    db[2u * P.ns + s] = P.omega_d * e2 + P.one_m_omega_d * sd2;
+    st_plane(db, s, P.omega_d * e0 + P.one_m_omega_d * sd0);
+    st_plane(db + (size_t)P.ns, s, P.omega_d * e1 + P.one_m_omega_d * sd1);
+    st_plane(db + (size_t)2 * P.ns, s, P.omega_d * e2 + P.one_m_omega_d * sd2);
+}
+
+template <typename R, bool Dev>
+__global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__ aux,
+                                                R *__restrict__ db)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    dipole_site<R, Dev>(P, aux, db, s);
 }
 
 // --------------------------------------------------------------------------
@@ -238,7 +253,7 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             if (CY[i] != 0) { frb[1] = frb[1] + green_r * R(CY[i]); fbr[1] = fbr[1] + green_b * R(CY[i]); }
             if (CZ[i] != 0) { frb[2] = frb[2] + green_r * R(CZ[i]); fbr[2] = fbr[2] + green_b * R(CZ[i]); }
         }
-        R psr = aux[s], psb = aux[P.ns + s];
+        R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s);
         if constexpr (Dev) {
             psr = P.psi0[0] + psr;
             psb = P.psi0[1] + psb;
@@ -250,7 +265,8 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
         }
     } else {
         R fcs[3] = {R(0), R(0), R(0)}, fsc[3] = {R(0), R(0), R(0)}, fss[3] = {R(0), R(0), R(0)};
-        R dbx0 = db[s], dbx1 = db[P.ns + s], dbx2 = db[2u * P.ns + s];
+        R dbx0 = ld_plane(db, s), dbx1 = ld_plane(plane_of(db, 1u, P.ns), s),
+          dbx2 = ld_plane(plane_of(db, 2u, P.ns), s);
         const R dpsi_br = Dev ? P.psi0[1] - P.psi0[0] : R(0);
 #pragma unroll
         for (int i = 0; i < 18; i++) {
@@ -284,7 +300,8 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
             fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
         }
-        R psr = aux[s], psb = aux[P.ns + s], pss = aux[2u * P.ns + s];
+        R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s),
+          pss = ld_plane(plane_of(aux, 2u, P.ns), s);
         if constexpr (Dev) {
             psr = P.psi0[0] + psr;
             psb = P.psi0[1] + psb;
@@ -393,14 +410,13 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
 template <typename R, bool Dev, int C, bool LE>
-__global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
-                                                 const R *__restrict__ face_pc,
-                                                 const R *__restrict__ aux,
-                                                 const R *__restrict__ face_aux,
-                                                 const R *__restrict__ db, R *__restrict__ fout)
+__device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
+                                             const R *__restrict__ face_pc,
+                                             const R *__restrict__ aux,
+                                             const R *__restrict__ face_aux,
+                                             const R *__restrict__ db, R *__restrict__ fout,
+                                             uint32_t s)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.ns) return;
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R n[C][19];
@@ -411,6 +427,18 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
     R F[C][3];
     sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
     bgk_site<R, Dev, C>(P, n, F, x, y, z, s, fout);
+}
+
+template <typename R, bool Dev, int C, bool LE>
+__global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
+                                                 const R *__restrict__ face_pc,
+                                                 const R *__restrict__ aux,
+                                                 const R *__restrict__ face_aux,
+                                                 const R *__restrict__ db, R *__restrict__ fout)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    collide_site<R, Dev, C, LE>(P, f, face_pc, aux, face_aux, db, fout, s);
 }
 
 // --------------------------------------------------------------------------

@@ -13,14 +13,17 @@ SHAPES = [(64, 4), (32, 8), (64, 8), (32, 16), (128, 4), (32, 4)]
 
 
 def fused_shapes(precision):
+    """(mode, tx, ty) of the one-pass kernel that exist for this form:
+    mode 1 stages the look-ahead plane in LDS, mode 2 keeps it in registers."""
     g = Lattice(8, 8, 8, ncomp=2, precision=precision)
     ok = []
-    for tx, ty in SHAPES:
-        try:
-            g.set_plan(1, tx, ty, 4)
-            ok.append((tx, ty))
-        except LB3DError:
-            pass
+    for mode in (1, 2):
+        for tx, ty in SHAPES:
+            try:
+                g.set_plan(mode, tx, ty, 4)
+                ok.append((mode, tx, ty))
+            except LB3DError:
+                pass
     return ok
 
 
@@ -35,13 +38,14 @@ def test_fused_equals_two_pass_bitwise(precision, dims, le):
     ref.set_plan(0)
     ref.time_loop(9)
     want = ref.populations()
-    for tx, ty in fused_shapes(precision):
+    for mode, tx, ty in fused_shapes(precision):
         for lz in (1, 4, dims[2]):
             _, g = pair(*dims, **kw)
-            g.set_plan(1, tx, ty, lz)
+            g.set_plan(mode, tx, ty, lz)
             g.time_loop(9)
             got = g.populations()
-            assert np.array_equal(got, want), f"{dims} le={le} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
+            assert np.array_equal(got, want), \
+                f"{dims} le={le} mode {mode} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
 
 
 def test_fused_fp64_matches_oracle_32cubed_100_steps():
