@@ -171,6 +171,42 @@ int lb3d_gpu_profile_steps(lb3d_ctx *ctx, int32_t nsteps, int32_t max_kernels,
                            const char **names, double *ms, int32_t *nkernels,
                            double *ms_per_step);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensemble (not a reference routine: the reference runs one lattice
+ * per job).  Many independent periodic lattices of uneven sizes -- parameter
+ * sweeps, seeds -- advanced together: each timestep is one persistent launch
+ * per phase over all lattices, replayed from a captured HIP graph.  All
+ * lattices share ncomp and precision; every other parameter is per lattice.
+ */
+typedef struct lb3d_batch lb3d_batch;
+
+int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **out);
+int lb3d_gpu_batch_destroy(lb3d_batch *b);
+/* State of lattice k: seeded noise (as lb3d_gpu_init_noise) or the
+ * reference's AoS lattice (as lb3d_gpu_upload_sites).  Both reset nt to 0. */
+int lb3d_gpu_batch_init_noise(lb3d_batch *b, int32_t k, uint64_t seed, const double rho_mean[3],
+                              const double amp[3], int32_t dipoles);
+int lb3d_gpu_batch_upload_sites(lb3d_batch *b, int32_t k, const double *whole_N,
+                                int32_t halo_extent, int32_t site_doubles);
+/* nsteps timesteps of every lattice. */
+int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps);
+int lb3d_gpu_batch_get_nt(lb3d_batch *b, int32_t *nt);
+int lb3d_gpu_batch_synchronize(lb3d_batch *b);
+/* Populations of lattice k in FP64 (layout as lb3d_gpu_get_populations). */
+int lb3d_gpu_batch_get_populations(lb3d_batch *b, int32_t k, double *out);
+/* Sanity reductions of lattice k (as lb3d_gpu_diagnostics). */
+int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double momentum[3],
+                               int64_t counts[3]);
+/* Plan: persistent launches with nwg workgroups (sequential = 0), or one
+ * lattice after another with the single-lattice kernels (sequential = 1). */
+int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg);
+/* Autotuner: times the sequential plan and persistent grid sizes on the
+ * batch's own state (unchanged) and keeps the fastest. */
+int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, int32_t report_len);
+/* Mean HIP-event duration of the moments, dipole and collide phases. */
+int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[3],
+                                 double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

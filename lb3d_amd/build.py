@@ -23,8 +23,9 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas"]
 UNITS = [
     ("lb3d_gpu.hip", ["-ffp-contract=off"]),
+    ("lb3d_batch.hip", ["-ffp-contract=off"]),
     ("lb3d_fp64.hip", ["-ffp-contract=off"]),
-    ("lb3d_fp32.hip", ["-ffp-contract=fast"]),
+    ("lb3d_fp32.hip", ["-ffp-contract=on"]),
 ]
 
 
@@ -33,7 +34,7 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_hip(jobs=3):
+def build_hip(jobs=4):
     os.makedirs(BUILD, exist_ok=True)
     procs, objs = [], []
     for src, extra in UNITS:

@@ -1,0 +1,426 @@
+// lb3d_batch.hip -- the batched ensemble of include/lb3d_gpu.h: many
+// independent periodic lattices of uneven sizes (parameter sweeps, seeds)
+// advanced together.  All lattices live in slices of shared buffers; one
+// timestep is the moments, dipole and collide phases, each a single
+// persistent launch over a work list of (lattice, 256-site tile) items, and
+// whole steps are replayed from a captured HIP graph.  The persistent grid
+// size is chosen by lb3d_gpu_batch_autotune.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lb3d_kernels.h"
+#include "lb3d_solver.h"
+
+using namespace lb3d;
+
+#define BTRY(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_error(LB3D_EHIP, (std::string(#expr) + ": " + hipGetErrorString(e_)).c_str()); \
+    } while (0)
+
+struct lb3d_batch {
+    int nlat = 0, C = 2, prec = LB3D_FP64;
+    std::vector<lb3d_params> p;
+    std::vector<Launcher *> L;  // per-lattice kernels (init, diagnostics, sequential plan)
+    BatchLauncher *B = nullptr;
+    size_t rs = 8;
+    std::vector<uint64_t> f_off, aux_off, db_off, ns;
+    void *f[2] = {nullptr, nullptr};
+    void *aux = nullptr, *db = nullptr, *lat = nullptr, *items = nullptr;
+    double *partials = nullptr;
+    uint32_t nitems = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int cur = 0, nt = 0;
+    unsigned nwg = 2048;     // persistent grid size
+    bool sequential = false; // one lattice after another with the single-lattice kernels
+};
+
+namespace {
+
+uint64_t round64(uint64_t v) { return (v + 63) & ~uint64_t(63); }
+
+DevBufs view(const lb3d_batch *b, int k)
+{
+    DevBufs v;
+    for (int j = 0; j < 2; j++) v.f[j] = (char *)b->f[j] + b->f_off[k] * b->rs;
+    v.aux = (char *)b->aux + b->aux_off[k] * b->rs;
+    v.db = b->C == 3 ? (char *)b->db + b->db_off[k] * b->rs : nullptr;
+    v.cur = b->cur;
+    return v;
+}
+
+void enqueue_batch_step(lb3d_batch *b)
+{
+    if (b->sequential) {
+        for (int k = 0; k < b->nlat; k++) {
+            DevBufs v = view(b, k);
+            b->L[k]->moments(v, b->stream);
+            b->L[k]->dipole(v, b->stream);
+            b->L[k]->collide(v, b->stream);
+        }
+    } else {
+        b->B->step(b->lat, b->items, b->nitems, b->nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db, b->C,
+                   b->stream);
+    }
+    b->cur = 1 - b->cur;
+}
+
+int drop_graph(lb3d_batch *b)
+{
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return LB3D_OK;
+}
+
+int ensure_graph(lb3d_batch *b)
+{
+    if (b->graph) return LB3D_OK;
+    int saved = b->cur;
+    b->cur = 0;
+    hipGraph_t g = nullptr;
+    BTRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_batch_step(b);
+    enqueue_batch_step(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->cur = saved;
+    if (e != hipSuccess) return set_error(LB3D_EHIP, (std::string("hipStreamEndCapture: ") + hipGetErrorString(e)).c_str());
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return set_error(LB3D_EHIP, (std::string("hipGraphInstantiate: ") + hipGetErrorString(e)).c_str());
+    }
+    return LB3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb3d_gpu_batch_destroy(lb3d_batch *b);
+
+int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **out)
+{
+    clear_error();
+    if (!out || !params || nlat < 1) return set_error(LB3D_EINVAL, "need nlat >= 1, params and out");
+    *out = nullptr;
+    const lb3d_params &p0 = params[0];
+    for (int k = 0; k < nlat; k++) {
+        const lb3d_params &q = params[k];
+        if (q.ncomp != p0.ncomp || q.precision != p0.precision)
+            return set_error(LB3D_EINVAL, "all lattices of a batch share ncomp and precision");
+        if (q.boundary != LB3D_BC_PERIODIC) return set_error(LB3D_EINVAL, "batched lattices are periodic");
+        lb3d_ctx *probe = nullptr;
+        (void)probe;
+        if (q.ncomp != 2 && q.ncomp != 3) return set_error(LB3D_EINVAL, "ncomp must be 2 or 3");
+        if (q.bdist != 0 && q.bdist != 2) return set_error(LB3D_EINVAL, "bdist must be 0 or 2");
+        if (q.precision != LB3D_FP64 && q.precision != LB3D_FP32_DEV)
+            return set_error(LB3D_EINVAL, "precision must be LB3D_FP64 or LB3D_FP32_DEV");
+        if (q.nx < 2 || q.ny < 1 || q.nz < 1) return set_error(LB3D_EINVAL, "lattice extent must be nx>=2, ny>=1, nz>=1");
+        for (int c = 0; c < q.ncomp; c++)
+            if (!(q.tau[c] > 0.0) || !(q.amass[c] > 0.0)) return set_error(LB3D_EINVAL, "tau and amass must be positive");
+        if (q.ncomp == 3 && !(q.tau_d > 0.0)) return set_error(LB3D_EINVAL, "tau_d must be positive");
+    }
+    lb3d_batch *b = new lb3d_batch();
+    b->nlat = nlat;
+    b->C = p0.ncomp;
+    b->prec = p0.precision;
+    b->rs = b->prec == LB3D_FP64 ? 8 : 4;
+    b->B = b->prec == LB3D_FP64 ? make_batch_launcher_fp64() : make_batch_launcher_fp32();
+    const int naux = b->C == 3 ? NAUX3 : NAUX2;
+    uint64_t fo = 0, ao = 0, dbo = 0, maxns = 0;
+    std::vector<uint2> items;
+    for (int k = 0; k < nlat; k++) {
+        b->p.push_back(params[k]);
+        b->L.push_back(b->prec == LB3D_FP64 ? make_launcher_fp64(params[k]) : make_launcher_fp32(params[k]));
+        uint64_t n = (uint64_t)params[k].nx * params[k].ny * params[k].nz;
+        b->ns.push_back(n);
+        b->f_off.push_back(fo);
+        b->aux_off.push_back(ao);
+        b->db_off.push_back(dbo);
+        fo = round64(fo + (uint64_t)b->C * 19 * n);
+        ao = round64(ao + (uint64_t)naux * n);
+        dbo = round64(dbo + (b->C == 3 ? 3 * n : 0));
+        if (n > maxns) maxns = n;
+        if ((uint64_t)b->C * 19 * n >= 4294967296ull) {
+            lb3d_gpu_batch_destroy(b);
+            return set_error(LB3D_EINVAL, "a lattice is too large for 32-bit population indexing");
+        }
+        for (uint64_t s0 = 0; s0 < n; s0 += BLOCK) items.push_back(make_uint2((unsigned)k, (unsigned)s0));
+    }
+    b->nitems = (uint32_t)items.size();
+    std::vector<unsigned char> lat((size_t)nlat * b->B->lat_bytes());
+    for (int k = 0; k < nlat; k++)
+        b->B->fill_lat(lat.data() + (size_t)k * b->B->lat_bytes(), params[k], b->f_off[k], b->aux_off[k], b->db_off[k]);
+    auto fail_destroy = [&](const char *what, hipError_t e) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        lb3d_gpu_batch_destroy(b);
+        return set_error(LB3D_ENOMEM, m.c_str());
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail_destroy("hipStreamCreate", e);
+    for (int j = 0; j < 2; j++)
+        if ((e = hipMalloc(&b->f[j], fo * b->rs)) != hipSuccess) return fail_destroy("hipMalloc populations", e);
+    if ((e = hipMalloc(&b->aux, ao * b->rs)) != hipSuccess) return fail_destroy("hipMalloc aux", e);
+    if (b->C == 3 && (e = hipMalloc(&b->db, dbo * b->rs)) != hipSuccess) return fail_destroy("hipMalloc db", e);
+    if ((e = hipMalloc(&b->lat, lat.size())) != hipSuccess) return fail_destroy("hipMalloc table", e);
+    if ((e = hipMalloc(&b->items, items.size() * sizeof(uint2))) != hipSuccess) return fail_destroy("hipMalloc items", e);
+    if ((e = hipMalloc((void **)&b->partials, ((maxns + BLOCK - 1) / BLOCK) * NDIAG * sizeof(double))) != hipSuccess)
+        return fail_destroy("hipMalloc partials", e);
+    if ((e = hipMemcpy(b->lat, lat.data(), lat.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->items, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_destroy("hipMemcpy", e);
+    b->nwg = b->nitems < 2048 ? b->nitems : 2048;
+    *out = b;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_destroy(lb3d_batch *b)
+{
+    if (!b) return LB3D_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (int j = 0; j < 2; j++)
+        if (b->f[j]) (void)hipFree(b->f[j]);
+    for (void *q : {b->aux, b->db, b->lat, b->items, (void *)b->partials})
+        if (q) (void)hipFree(q);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (Launcher *l : b->L) delete l;
+    delete b->B;
+    delete b;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_init_noise(lb3d_batch *b, int32_t k, uint64_t seed, const double rho_mean[3],
+                              const double amp[3], int32_t dipoles)
+{
+    clear_error();
+    if (!b || k < 0 || k >= b->nlat || !rho_mean || !amp) return set_error(LB3D_EINVAL, "bad argument");
+    b->L[k]->init_noise(view(b, k), seed, rho_mean, amp, dipoles, b->stream);
+    BTRY(hipGetLastError());
+    BTRY(hipStreamSynchronize(b->stream));
+    b->nt = 0;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_upload_sites(lb3d_batch *b, int32_t k, const double *whole_N, int32_t halo, int32_t sd)
+{
+    clear_error();
+    if (!b || k < 0 || k >= b->nlat || !whole_N) return set_error(LB3D_EINVAL, "bad argument");
+    std::vector<unsigned char> fb, dbb;
+    int rc = aos_to_device_layout(b->p[k], whole_N, halo, sd, fb, dbb);
+    if (rc) return rc;
+    DevBufs v = view(b, k);
+    BTRY(hipMemcpyAsync(v.f[b->cur], fb.data(), fb.size(), hipMemcpyHostToDevice, b->stream));
+    if (b->C == 3) BTRY(hipMemcpyAsync(v.db, dbb.data(), dbb.size(), hipMemcpyHostToDevice, b->stream));
+    BTRY(hipStreamSynchronize(b->stream));
+    b->nt = 0;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_get_populations(lb3d_batch *b, int32_t k, double *out)
+{
+    clear_error();
+    if (!b || k < 0 || k >= b->nlat || !out) return set_error(LB3D_EINVAL, "bad argument");
+    size_t bytes = (size_t)b->C * 19 * b->ns[k] * b->rs;
+    std::vector<unsigned char> h(bytes);
+    DevBufs v = view(b, k);
+    BTRY(hipMemcpyAsync(h.data(), v.f[b->cur], bytes, hipMemcpyDeviceToHost, b->stream));
+    BTRY(hipStreamSynchronize(b->stream));
+    device_layout_to_pops(b->p[k], h.data(), out);
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps)
+{
+    clear_error();
+    if (!b || nsteps < 0) return set_error(LB3D_EINVAL, "bad argument");
+    if (nsteps == 0) return LB3D_OK;
+    int rc = ensure_graph(b);
+    if (rc) return rc;
+    int left = nsteps;
+    if (b->cur == 1) {
+        enqueue_batch_step(b);
+        left--;
+    }
+    for (; left >= 2; left -= 2) BTRY(hipGraphLaunch(b->graph, b->stream));
+    if (left == 1) enqueue_batch_step(b);
+    BTRY(hipGetLastError());
+    b->nt += nsteps;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_synchronize(lb3d_batch *b)
+{
+    clear_error();
+    if (!b) return set_error(LB3D_EINVAL, "NULL batch");
+    BTRY(hipStreamSynchronize(b->stream));
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double momentum[3], int64_t counts[3])
+{
+    clear_error();
+    if (!b || k < 0 || k >= b->nlat || !mass || !momentum || !counts) return set_error(LB3D_EINVAL, "bad argument");
+    b->L[k]->diagnostics(view(b, k), b->partials, b->stream);
+    BTRY(hipGetLastError());
+    unsigned nb = b->L[k]->diag_blocks();
+    std::vector<double> h((size_t)nb * NDIAG);
+    BTRY(hipMemcpyAsync(h.data(), b->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    BTRY(hipStreamSynchronize(b->stream));
+    double acc[NDIAG] = {0};
+    for (unsigned q = 0; q < nb; q++)
+        for (int j = 0; j < NDIAG; j++) acc[j] = acc[j] + h[(size_t)q * NDIAG + j];
+    for (int j = 0; j < 3; j++) {
+        mass[j] = j < b->C ? acc[j] : 0.0;
+        momentum[j] = acc[3 + j];
+        counts[j] = (int64_t)llround(acc[6 + j]);
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg)
+{
+    clear_error();
+    if (!b) return set_error(LB3D_EINVAL, "NULL batch");
+    if (!sequential && nwg < 1) return set_error(LB3D_EINVAL, "nwg must be >= 1");
+    b->sequential = sequential != 0;
+    if (!b->sequential) b->nwg = (unsigned)nwg < b->nitems ? (unsigned)nwg : b->nitems;
+    return drop_graph(b);
+}
+
+// Times the persistent launches over grid sizes, and the lattice-by-lattice
+// sequence, on the batch's own state; trial steps write f[1-cur] without
+// swapping (the ternary dipole field is saved and restored).
+int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, int32_t report_len)
+{
+    clear_error();
+    if (!b) return set_error(LB3D_EINVAL, "NULL batch");
+    if (trial_steps < 1) trial_steps = 1;
+    struct Cand {
+        bool seq;
+        unsigned nwg;
+    };
+    std::vector<Cand> cands = {{true, 0}};
+    for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
+        if (g < b->nitems) cands.push_back({false, g});
+    cands.push_back({false, b->nitems});
+    void *db_save = nullptr;
+    size_t db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
+    if (db_bytes) {
+        BTRY(hipMalloc(&db_save, db_bytes));
+        BTRY(hipMemcpyAsync(db_save, b->db, db_bytes, hipMemcpyDeviceToDevice, b->stream));
+    }
+    hipEvent_t e0, e1;
+    BTRY(hipEventCreate(&e0));
+    BTRY(hipEventCreate(&e1));
+    bool seq0 = b->sequential;
+    unsigned nwg0 = b->nwg;
+    double best = 1e300;
+    Cand bc = {seq0, nwg0};
+    std::string rep;
+    for (const Cand &c : cands) {
+        b->sequential = c.seq;
+        b->nwg = c.nwg;
+        auto trial = [&]() {
+            enqueue_batch_step(b);
+            b->cur = 1 - b->cur;  // undo the swap: the state stays in f[cur]
+        };
+        trial();
+        BTRY(hipEventRecord(e0, b->stream));
+        for (int t = 0; t < trial_steps; t++) trial();
+        BTRY(hipEventRecord(e1, b->stream));
+        BTRY(hipGetLastError());
+        BTRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BTRY(hipEventElapsedTime(&ms, e0, e1));
+        double per = ms / trial_steps;
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "%s nwg=%u %.4f ms\n", c.seq ? "sequential" : "persistent", c.nwg, per);
+        rep += buf;
+        if (per < best) {
+            best = per;
+            bc = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (db_save) {
+        BTRY(hipMemcpyAsync(b->db, db_save, db_bytes, hipMemcpyDeviceToDevice, b->stream));
+        BTRY(hipStreamSynchronize(b->stream));
+        (void)hipFree(db_save);
+    }
+    b->sequential = bc.seq;
+    b->nwg = bc.nwg;
+    drop_graph(b);
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u %.4f ms\n", bc.seq ? "sequential" : "persistent", bc.nwg, best);
+    rep += buf;
+    if (report && report_len > 0) {
+        std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return LB3D_OK;
+}
+
+// Mean duration per launch of the three phases (persistent plan) over
+// nsteps timesteps, HIP events on the batch's stream.
+int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[3], double *ms_per_step)
+{
+    clear_error();
+    if (!b || nsteps < 1 || !ms_phase || !ms_per_step) return set_error(LB3D_EINVAL, "bad argument");
+    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    for (auto &e : ev) BTRY(hipEventCreate(&e));
+    for (int st = 0; st < nsteps; st++) {
+        BTRY(hipEventRecord(ev[(size_t)st * 4], b->stream));
+        for (int ph = 0; ph < 3; ph++) {
+            if (b->sequential) {
+                for (int k = 0; k < b->nlat; k++) {
+                    DevBufs v = view(b, k);
+                    if (ph == 0) b->L[k]->moments(v, b->stream);
+                    if (ph == 1) b->L[k]->dipole(v, b->stream);
+                    if (ph == 2) b->L[k]->collide(v, b->stream);
+                }
+            } else {
+                b->B->phase(ph, b->lat, b->items, b->nitems, b->nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db,
+                            b->C, b->stream);
+            }
+            BTRY(hipEventRecord(ev[(size_t)st * 4 + 1 + ph], b->stream));
+        }
+        b->cur = 1 - b->cur;
+    }
+    BTRY(hipGetLastError());
+    BTRY(hipStreamSynchronize(b->stream));
+    double tot[3] = {0, 0, 0}, all = 0;
+    for (int st = 0; st < nsteps; st++) {
+        for (int ph = 0; ph < 3; ph++) {
+            float m = 0.f;
+            BTRY(hipEventElapsedTime(&m, ev[(size_t)st * 4 + ph], ev[(size_t)st * 4 + ph + 1]));
+            tot[ph] += m;
+            all += m;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int ph = 0; ph < 3; ph++) ms_phase[ph] = tot[ph] / nsteps;
+    *ms_per_step = all / nsteps;
+    b->nt += nsteps;
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_get_nt(lb3d_batch *b, int32_t *nt)
+{
+    if (!b || !nt) return set_error(LB3D_EINVAL, "bad argument");
+    *nt = b->nt;
+    return LB3D_OK;
+}
+
+}  // extern "C"

@@ -4,4 +4,5 @@
 
 namespace lb3d {
 Launcher *make_launcher_fp32(const lb3d_params &p) { return new LauncherT<float, true>(p); }
+BatchLauncher *make_batch_launcher_fp32() { return new BatchLauncherT<float, true>(); }
 }  // namespace lb3d

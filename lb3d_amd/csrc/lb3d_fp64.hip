@@ -4,4 +4,5 @@
 
 namespace lb3d {
 Launcher *make_launcher_fp64(const lb3d_params &p) { return new LauncherT<double, false>(p); }
+BatchLauncher *make_batch_launcher_fp64() { return new BatchLauncherT<double, false>(); }
 }  // namespace lb3d

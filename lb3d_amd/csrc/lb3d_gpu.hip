@@ -92,6 +92,73 @@ static int check_params(const lb3d_params *p)
     return LB3D_OK;
 }
 
+namespace lb3d {
+int set_error(int code, const char *msg) { return fail(code, msg); }
+void clear_error() { g_err.clear(); }
+double rho0_of(const lb3d_params &p, int k) { return p.rho0[k] > 0.0 ? p.rho0[k] : 0.7; }
+
+int aos_to_device_layout(const lb3d_params &p, const double *N, int h, int sd,
+                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes)
+{
+    const int C = p.ncomp;
+    if (sd != (C == 3 ? 79 : 49)) return fail(LB3D_EINVAL, "site_doubles must be 79 (ternary) or 49 (binary)");
+    if (h < 1) return fail(LB3D_EINVAL, "halo_extent must be >= 1");
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t ns = (size_t)nx * ny * nz;
+    const size_t lx = nx + 2 * h, ly = ny + 2 * h;
+    const int rock_off = C == 3 ? 78 : 48;
+    const bool dev = p.precision == LB3D_FP32_DEV;
+    const size_t rs = dev ? 4 : 8;
+    f_bytes.assign((size_t)C * 19 * ns * rs, 0);
+    db_bytes.assign(C == 3 ? 3 * ns * rs : 0, 0);
+    float *ff = (float *)f_bytes.data();
+    double *fd = (double *)f_bytes.data();
+    float *dbf = (float *)db_bytes.data();
+    double *dbd = (double *)db_bytes.data();
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const double *S = N + ((size_t)(x + h) + lx * ((size_t)(y + h) + ly * (size_t)(z + h))) * sd;
+                size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
+                if (S[rock_off] != 0.0) return fail(LB3D_EINVAL, "rock sites are not supported (rock_state must be 0)");
+                for (int k = 0; k < C; k++)
+                    for (int i = 0; i < 19; i++) {
+                        double v = S[24 * k + i];
+                        size_t q = ((size_t)k * 19 + i) * ns + s;
+                        if (dev)
+                            ff[q] = (float)(v - (i < 18 ? 1.0 / 36.0 : 1.0 / 3.0) * rho0_of(p, k));
+                        else
+                            fd[q] = v;
+                    }
+                if (C == 3)
+                    for (int k = 0; k < 3; k++) {
+                        if (dev)
+                            dbf[k * ns + s] = (float)S[75 + k];
+                        else
+                            dbd[k * ns + s] = S[75 + k];
+                    }
+            }
+    return LB3D_OK;
+}
+
+void device_layout_to_pops(const lb3d_params &p, const unsigned char *fb, double *out)
+{
+    const int C = p.ncomp;
+    const size_t ns = (size_t)p.nx * p.ny * p.nz;
+    if (p.precision == LB3D_FP32_DEV) {
+        const float *h = (const float *)fb;
+        for (int k = 0; k < C; k++)
+            for (int i = 0; i < 19; i++) {
+                double base = (i < 18 ? 1.0 / 36.0 : 1.0 / 3.0) * rho0_of(p, k);
+                size_t off = ((size_t)k * 19 + i) * ns;
+                for (size_t q = 0; q < ns; q++) out[off + q] = base + (double)h[off + q];
+            }
+    } else {
+        std::memcpy(out, fb, (size_t)C * 19 * ns * sizeof(double));
+    }
+}
+}  // namespace lb3d
+
 extern "C" {
 
 int lb3d_gpu_abi_version(void) { return LB3D_ABI_VERSION; }
@@ -188,54 +255,20 @@ static int pull_le(lb3d_ctx *c)
 }
 
 static bool is_dev(const lb3d_ctx *c) { return c->p.precision == LB3D_FP32_DEV; }
-static double rho0_of(const lb3d_ctx *c, int k) { return c->p.rho0[k] > 0.0 ? c->p.rho0[k] : 0.7; }
+static double rho0_of(const lb3d_ctx *c, int k) { return lb3d::rho0_of(c->p, k); }
 static double w_of(int i) { return i < 18 ? 1.0 / 36.0 : 1.0 / 3.0; }
 
 int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, int32_t nt0)
 {
     g_err.clear();
     if (!c || !N) return fail(LB3D_EINVAL, "NULL argument");
-    const int C = c->p.ncomp;
-    if (sd != (C == 3 ? 79 : 49)) return fail(LB3D_EINVAL, "site_doubles must be 79 (ternary) or 49 (binary)");
-    if (h < 1) return fail(LB3D_EINVAL, "halo_extent must be >= 1");
-    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
-    const size_t ns = (size_t)nx * ny * nz;
-    const size_t lx = nx + 2 * h, ly = ny + 2 * h;
-    const int rock_off = C == 3 ? 78 : 48;
-    const bool dev = is_dev(c);
-    std::vector<double> fd(dev ? 0 : (size_t)C * 19 * ns);
-    std::vector<float> ff(dev ? (size_t)C * 19 * ns : 0);
-    std::vector<double> dbd(!dev && C == 3 ? 3 * ns : 0);
-    std::vector<float> dbf(dev && C == 3 ? 3 * ns : 0);
-    for (int z = 0; z < nz; z++)
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) {
-                const double *S = N + ((size_t)(x + h) + lx * ((size_t)(y + h) + ly * (size_t)(z + h))) * sd;
-                size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
-                if (S[rock_off] != 0.0) return fail(LB3D_EINVAL, "rock sites are not supported (rock_state must be 0)");
-                for (int k = 0; k < C; k++)
-                    for (int i = 0; i < 19; i++) {
-                        double v = S[24 * k + i];
-                        size_t q = ((size_t)k * 19 + i) * ns + s;
-                        if (dev)
-                            ff[q] = (float)(v - w_of(i) * rho0_of(c, k));
-                        else
-                            fd[q] = v;
-                    }
-                if (C == 3)
-                    for (int k = 0; k < 3; k++) {
-                        if (dev)
-                            dbf[k * ns + s] = (float)S[75 + k];
-                        else
-                            dbd[k * ns + s] = S[75 + k];
-                    }
-            }
+    std::vector<unsigned char> fb, dbb;
+    int rc = aos_to_device_layout(c->p, N, h, sd, fb, dbb);
+    if (rc) return rc;
     c->b.cur = 0;
-    HIP_TRY(hipMemcpyAsync(c->b.f[0], dev ? (const void *)ff.data() : (const void *)fd.data(), c->f_bytes,
-                           hipMemcpyHostToDevice, c->stream));
-    if (C == 3)
-        HIP_TRY(hipMemcpyAsync(c->b.db, dev ? (const void *)dbf.data() : (const void *)dbd.data(), c->db_bytes,
-                               hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->b.f[0], fb.data(), c->f_bytes, hipMemcpyHostToDevice, c->stream));
+    if (c->p.ncomp == 3)
+        HIP_TRY(hipMemcpyAsync(c->b.db, dbb.data(), c->db_bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->nt = nt0;
     c->host_le.nt = nt0;
@@ -266,22 +299,10 @@ int lb3d_gpu_get_populations(lb3d_ctx *c, double *out)
 {
     g_err.clear();
     if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
-    const int C = c->p.ncomp;
-    const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
-    if (is_dev(c)) {
-        std::vector<float> h((size_t)C * 19 * ns);
-        HIP_TRY(hipMemcpyAsync(h.data(), c->b.f[c->b.cur], c->f_bytes, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        for (int k = 0; k < C; k++)
-            for (int i = 0; i < 19; i++) {
-                double base = w_of(i) * rho0_of(c, k);
-                size_t off = ((size_t)k * 19 + i) * ns;
-                for (size_t s = 0; s < ns; s++) out[off + s] = base + (double)h[off + s];
-            }
-    } else {
-        HIP_TRY(hipMemcpyAsync(out, c->b.f[c->b.cur], c->f_bytes, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    std::vector<unsigned char> h(c->f_bytes);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->b.f[c->b.cur], c->f_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    device_layout_to_pops(c->p, h.data(), out);
     return LB3D_OK;
 }
 

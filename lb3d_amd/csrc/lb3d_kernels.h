@@ -796,6 +796,74 @@ __global__ __launch_bounds__(128) void k_le_faces(Phys<R> P, const LEState *__re
 // le_neighbours (code/lb3d_bc_leesedwards.F90:86-140, inv_fluid = 5) with the
 // stage-6 update shear_u = shear_tmp*cos(shear_omega*nt) (code/lb3d_bc.F90:110),
 // in FP64 on the device so that the step sequence can be replayed from a graph.
+// --------------------------------------------------------------------------
+// Batched ensemble: many independent periodic lattices of uneven sizes in the
+// same launches.  Each lattice has its own parameters and a slice of the
+// shared buffers; the work list holds (lattice, first site) tiles of 256
+// sites.  Persistent grid: a fixed number of workgroups strides over the list.
+template <typename R>
+struct BatchLat {
+    Phys<R> P;
+    uint64_t f_off, aux_off, db_off;  // element offsets of this lattice's slices
+};
+
+__device__ __forceinline__ uint2 batch_item(const uint2 *__restrict__ items, uint32_t it)
+{
+    uint2 w = items[it];
+    w.x = __builtin_amdgcn_readfirstlane(w.x);
+    w.y = __builtin_amdgcn_readfirstlane(w.y);
+    return w;
+}
+
+template <typename R, bool Dev, int C>
+__global__ __launch_bounds__(256) void k_batch_moments(const BatchLat<R> *__restrict__ lat,
+                                                       const uint2 *__restrict__ items,
+                                                       uint32_t nitems, const R *__restrict__ f,
+                                                       const R *__restrict__ db,
+                                                       R *__restrict__ aux)
+{
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        uint2 w = batch_item(items, it);
+        const BatchLat<R> &L = lat[w.x];
+        uint32_t s = w.y + threadIdx.x;
+        if (s < L.P.ns)
+            moments_site<R, Dev, C, false>(L.P, f + L.f_off, nullptr, C == 3 ? db + L.db_off : nullptr,
+                                           aux + L.aux_off, s);
+    }
+}
+
+template <typename R, bool Dev>
+__global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restrict__ lat,
+                                                      const uint2 *__restrict__ items,
+                                                      uint32_t nitems, const R *__restrict__ aux,
+                                                      R *__restrict__ db)
+{
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        uint2 w = batch_item(items, it);
+        const BatchLat<R> &L = lat[w.x];
+        uint32_t s = w.y + threadIdx.x;
+        if (s < L.P.ns) dipole_site<R, Dev>(L.P, aux + L.aux_off, db + L.db_off, s);
+    }
+}
+
+template <typename R, bool Dev, int C>
+__global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__restrict__ lat,
+                                                       const uint2 *__restrict__ items,
+                                                       uint32_t nitems, const R *__restrict__ f,
+                                                       const R *__restrict__ aux,
+                                                       const R *__restrict__ db,
+                                                       R *__restrict__ fout)
+{
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        uint2 w = batch_item(items, it);
+        const BatchLat<R> &L = lat[w.x];
+        uint32_t s = w.y + threadIdx.x;
+        if (s < L.P.ns)
+            collide_site<R, Dev, C, false>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
+                                           C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
+    }
+}
+
 static __global__ void k_le_neighbours(LEState *L, int nz, int nt_set)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;

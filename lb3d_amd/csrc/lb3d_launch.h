@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstring>
 
 #include "lb3d_kernels.h"
 #include "lb3d_solver.h"
@@ -276,6 +277,48 @@ class LauncherT : public Launcher {
     Phys<R> P;
     int C;
     bool le;
+};
+
+template <typename R, bool Dev>
+class BatchLauncherT : public BatchLauncher {
+  public:
+    size_t lat_bytes() const override { return sizeof(BatchLat<R>); }
+    void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
+                  uint64_t db_off) const override
+    {
+        BatchLat<R> L{};
+        L.P = make_phys<R>(p);
+        L.f_off = f_off;
+        L.aux_off = aux_off;
+        L.db_off = db_off;
+        std::memcpy(dst, &L, sizeof(L));
+    }
+    void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
+               void *fout, void *aux, void *db, int ncomp, hipStream_t s) override
+    {
+        const BatchLat<R> *L = (const BatchLat<R> *)lat;
+        const uint2 *it = (const uint2 *)items;
+        if (ph == 0) {
+            if (ncomp == 3)
+                k_batch_moments<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)db, (R *)aux);
+            else
+                k_batch_moments<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, nullptr, (R *)aux);
+        } else if (ph == 1) {
+            if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db);
+        } else {
+            if (ncomp == 3)
+                k_batch_collide<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                  (const R *)db, (R *)fout);
+            else
+                k_batch_collide<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                  nullptr, (R *)fout);
+        }
+    }
+    void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,
+              void *aux, void *db, int ncomp, hipStream_t s) override
+    {
+        for (int ph = 0; ph < 3; ph++) phase(ph, lat, items, nitems, nwg, f, fout, aux, db, ncomp, s);
+    }
 };
 
 }  // namespace lb3d

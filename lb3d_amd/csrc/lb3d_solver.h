@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/lb3d_gpu.h"
 #include "lb3d_lattice.h"
 
@@ -46,8 +48,38 @@ constexpr int N_FUSED_SHAPES = sizeof(FUSED_SHAPES) / sizeof(FUSED_SHAPES[0]);
 constexpr int NDIAG = 9;  // mass[3], momentum[3], nan sites, negative sites, fast sites
 constexpr unsigned BLOCK = 256;
 
+// Batched ensemble kernels of one precision.  The lattice table (BatchLat<R>
+// per lattice) and the work list live in device memory.
+class BatchLauncher {
+  public:
+    virtual ~BatchLauncher() {}
+    virtual size_t lat_bytes() const = 0;   // sizeof(BatchLat<R>)
+    // host image of the lattice table entry k
+    virtual void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
+                          uint64_t db_off) const = 0;
+    virtual void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
+                      void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
+    // the three phases separately (for profiling): 0 moments, 1 dipole, 2 collide
+    virtual void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg,
+                       const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
+};
+
 Launcher *make_launcher_fp64(const lb3d_params &p);
 Launcher *make_launcher_fp32(const lb3d_params &p);
+BatchLauncher *make_batch_launcher_fp64();
+BatchLauncher *make_batch_launcher_fp32();
+
+// Error state of the C ABI (thread-local message, returns code).
+int set_error(int code, const char *msg);
+void clear_error();
+// Host conversion of the reference's AoS lattice (lbe_site records) into the
+// device layout of one lattice: populations (FP64, or FP32 deviations from
+// w_i*rho0) and the ternary dipole field.  Returns 0 or an LB3D_E* code.
+int aos_to_device_layout(const lb3d_params &p, const double *whole_N, int halo, int site_doubles,
+                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes);
+// The inverse for the populations: device bytes -> FP64 populations.
+void device_layout_to_pops(const lb3d_params &p, const unsigned char *f_bytes, double *out);
+double rho0_of(const lb3d_params &p, int c);
 
 // Parameters in the kernels' working type.
 template <typename R>

@@ -35,6 +35,10 @@ EXPORTS = [
     "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
     "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
     "lb3d_gpu_get_density", "lb3d_gpu_synchronize", "lb3d_gpu_autotune", "lb3d_gpu_set_plan",
+    "lb3d_gpu_batch_create", "lb3d_gpu_batch_destroy", "lb3d_gpu_batch_init_noise",
+    "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
+    "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
+    "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_autotune", "lb3d_gpu_batch_profile_steps",
 ]
 
 
@@ -95,6 +99,18 @@ def lib():
         L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip]
         L.lb3d_gpu_profile_steps.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_char_p), dp,
                                              ctypes.POINTER(ip), dp]
+        L.lb3d_gpu_batch_create.argtypes = [ip, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.lb3d_gpu_batch_destroy.argtypes = [vp]
+        L.lb3d_gpu_batch_init_noise.argtypes = [vp, ip, ctypes.c_uint64, dp, dp, ip]
+        L.lb3d_gpu_batch_upload_sites.argtypes = [vp, ip, dp, ip, ip]
+        L.lb3d_gpu_batch_run_steps.argtypes = [vp, ip]
+        L.lb3d_gpu_batch_get_nt.argtypes = [vp, ctypes.POINTER(ip)]
+        L.lb3d_gpu_batch_synchronize.argtypes = [vp]
+        L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
+        L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+        L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip]
+        L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
+        L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
         _lib = L
     return _lib
 
@@ -261,3 +277,87 @@ class Lattice:
         check(lib().lb3d_gpu_profile_steps(self._h, int(nsteps), mx, names, ms, ctypes.byref(nk),
                                            ctypes.byref(mps)), "profile_steps")
         return {names[i].decode(): ms[i] for i in range(nk.value)}, mps.value
+
+
+class Batch:
+    """An ensemble of independent periodic lattices advanced together
+    (lb3d_gpu_batch_* of include/lb3d_gpu.h).  `dims` is a list of (nx, ny, nz);
+    `per_lattice` optionally a list of keyword dicts (e.g. g_br) per lattice;
+    common keywords apply to all."""
+
+    def __init__(self, dims, per_lattice=None, **common):
+        self.dims = list(dims)
+        per = per_lattice or [{} for _ in self.dims]
+        arr = (Params * len(self.dims))()
+        for k, (d, extra) in enumerate(zip(self.dims, per)):
+            kw = dict(common)
+            kw.update(extra)
+            arr[k] = make_params(*d, **kw)
+        self.ncomp = arr[0].ncomp
+        self.site_doubles = 79 if self.ncomp == 3 else 49
+        h = ctypes.c_void_p()
+        check(lib().lb3d_gpu_batch_create(len(self.dims), arr, ctypes.byref(h)), "batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lb3d_gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_noise(self, k, seed, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True):
+        m = (ctypes.c_double * 3)(*rho_mean)
+        a = (ctypes.c_double * 3)(*amp)
+        check(lib().lb3d_gpu_batch_init_noise(self._h, k, ctypes.c_uint64(seed), m, a, 1 if dipoles else 0),
+              "batch_init_noise")
+
+    def upload_sites(self, k, whole_N, halo_extent=1):
+        a = np.ascontiguousarray(whole_N, dtype=np.float64)
+        check(lib().lb3d_gpu_batch_upload_sites(self._h, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                halo_extent, self.site_doubles), "batch_upload_sites")
+
+    def time_loop(self, nsteps):
+        check(lib().lb3d_gpu_batch_run_steps(self._h, int(nsteps)), "batch_run_steps")
+
+    def synchronize(self):
+        check(lib().lb3d_gpu_batch_synchronize(self._h), "batch_synchronize")
+
+    @property
+    def nt(self):
+        v = ctypes.c_int32()
+        check(lib().lb3d_gpu_batch_get_nt(self._h, ctypes.byref(v)), "batch_get_nt")
+        return v.value
+
+    def populations(self, k):
+        nx, ny, nz = self.dims[k]
+        out = np.empty((self.ncomp, 19, nz, ny, nx))
+        check(lib().lb3d_gpu_batch_get_populations(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "batch_get_populations")
+        return out
+
+    def sanity_check(self, k):
+        m = (ctypes.c_double * 3)()
+        p = (ctypes.c_double * 3)()
+        c = (ctypes.c_int64 * 3)()
+        check(lib().lb3d_gpu_batch_diagnostics(self._h, k, m, p, c), "batch_diagnostics")
+        return np.array(m[:]), np.array(p[:]), np.array(c[:])
+
+    def set_plan(self, sequential, nwg=2048):
+        check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg)), "batch_set_plan")
+
+    def autotune(self, trial_steps=3):
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().lb3d_gpu_batch_autotune(self._h, int(trial_steps), buf, len(buf)), "batch_autotune")
+        return buf.value.decode()
+
+    def profile_steps(self, nsteps):
+        ph = (ctypes.c_double * 3)()
+        mps = ctypes.c_double()
+        check(lib().lb3d_gpu_batch_profile_steps(self._h, int(nsteps), ph, ctypes.byref(mps)),
+              "batch_profile_steps")
+        return dict(moments=ph[0], dipole=ph[1], collide=ph[2]), mps.value
