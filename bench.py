@@ -60,6 +60,57 @@ def run_form(n, ncomp, precision, steps, warmup, profile_steps):
                 counts=counts.tolist())
 
 
+def ensemble_spec(nlat=48, seed=1):
+    """The batched ternary ensemble: cube edges uniform in {16, 24, ..., 96},
+    g_br uniform in [0.06, 0.12], noise seed 1000+k (numpy default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    edges = 16 + 8 * rng.integers(0, 11, nlat)
+    g_br = 0.06 + 0.06 * rng.random(nlat)
+    return [int(e) for e in edges], [float(g) for g in g_br]
+
+
+def run_ensemble(steps, warmup, seq_steps, nlat=48):
+    """FP32 batched persistent launches against the lattices run one after
+    another on the plain FP64 path."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64, Batch
+
+    edges, g_br = ensemble_spec(nlat)
+    dims = [(e, e, e) for e in edges]
+    per = [dict(g_br=g) for g in g_br]
+    common = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
+    sites = sum(e ** 3 for e in edges)
+    out = {}
+    for name, prec, nsteps in (("fp32_batched", LB3D_FP32_DEV, steps), ("fp64_sequential", LB3D_FP64, seq_steps)):
+        b = Batch(dims, per_lattice=per, precision=prec, **common)
+        for k in range(nlat):
+            b.init_noise(k, 1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+        if prec == LB3D_FP64:
+            b.set_plan(1)   # lattice after lattice, the single-lattice FP64 kernels
+            chosen = "sequential"
+        else:
+            chosen = b.autotune(3).strip().splitlines()[-1]
+        b.time_loop(warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.time_loop(nsteps)
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        ph, mps = b.profile_steps(3)
+        out[name] = dict(mlups=sites * nsteps / dt / 1e6, ms_per_step=1e3 * dt / nsteps, plan=chosen,
+                         phases_ms=ph, steps=nsteps)
+        print(f"[bench] ensemble {name}: {out[name]['mlups']:.1f} MLUPS {chosen} {ph}", file=sys.stderr, flush=True)
+        b.close()
+    B = bytes_per_site(3, True, 4)
+    r = out["fp32_batched"]
+    return dict(lattices=nlat, sites=sites, edges=edges, value=round(r["mlups"], 1), unit="MLUPS",
+                ms_per_step=round(r["ms_per_step"], 4), steps=r["steps"], plan=r["plan"],
+                step_frac=round(r["mlups"] * 1e6 * B / (HBM_PEAK_GBS * 1e9), 4), bytes_per_site=B,
+                phases_ms={k: round(v, 4) for k, v in r["phases_ms"].items()},
+                fp64_sequential=dict(value=round(out["fp64_sequential"]["mlups"], 1), unit="MLUPS",
+                                     steps=out["fp64_sequential"]["steps"]),
+                speedup_vs_fp64_sequential=round(r["mlups"] / out["fp64_sequential"]["mlups"], 3))
+
+
 def cpu_baseline(sample_n, sample_steps):
     from oracle.pyoracle import Oracle
 
@@ -85,6 +136,9 @@ def main():
     ap.add_argument("--cpu-sample-size", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--ensemble-fp64-steps", type=int, default=20)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -135,6 +189,9 @@ def main():
     fp64 = dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
                 step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
                 kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()}, plan=r64["plan"])
+    ens = None
+    if not args.no_ensemble:
+        ens = run_ensemble(args.ensemble_steps, 5, args.ensemble_fp64_steps)
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline oracle {args.cpu_sample_size}^3 x{args.cpu_sample_steps}", file=sys.stderr, flush=True)
@@ -162,6 +219,7 @@ def main():
         "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "accuracy_fp32_vs_fp64": acc,
         "kernels_ms": {k: round(v, 5) for k, v in r32["kernels"].items()},
+        "batched_ternary_ensemble": ens,
     }
     print(json.dumps(line), flush=True)
 

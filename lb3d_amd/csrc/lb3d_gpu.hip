@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -206,8 +207,8 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
         (e = hipMalloc((void **)&c->d_partials, (size_t)c->L->diag_blocks() * NDIAG * sizeof(double))) !=
             hipSuccess)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    if (p->ncomp == 2) {
-        // default one-pass plan; lb3d_gpu_autotune picks the measured best
+    if (p->ncomp == 2 && c->L->fused_supported(32, 4, true)) {
+        // default one-pass plan where it exists; lb3d_gpu_autotune picks the measured best
         c->plan.fused = true;
         c->plan.tx = c->L->fused_supported(64, 4, true) ? 64 : 32;
         c->plan.ty = 4;
@@ -485,7 +486,8 @@ int lb3d_gpu_advection(lb3d_ctx *c)
 static void launch_collision(lb3d_ctx *c)
 {
     if (c->plan.fused) {
-        c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream);
+        // plans are validated when set (lb3d_gpu_set_plan, autotune, create)
+        if (!c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream)) abort();
     } else {
         c->L->dipole(c->b, c->stream);
         c->L->collide(c->b, c->stream);
@@ -689,7 +691,8 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     q.fused = fused != 0;
     q.stage = fused != 2;  // fused = 2: look-ahead in registers
     if (q.fused) {
-        if (c->p.ncomp != 2) return fail(LB3D_EINVAL, "the one-pass kernel is binary only");
+        if (c->p.ncomp != 2 || c->p.bdist != 2)
+            return fail(LB3D_EINVAL, "the one-pass kernel is for binary lattices with bdist 2");
         bool ok = c->L->fused_supported(tx, ty, q.stage);
         if (!ok || lz < 1) return fail(LB3D_EINVAL, "no such tile shape");
         q.tx = tx;

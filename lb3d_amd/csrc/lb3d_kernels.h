@@ -326,7 +326,7 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
 // F: moments, body force, BGK relaxation towards rho*boltz(u_tilde, du),
 // code/lb3d_collision.F90:188-289 with lb3d_get_sc_u_eq (:560-587) and
 // lb3d_get_sc_du (:523-556).  Writes the post-collision populations of site s.
-template <typename R, bool Dev, int C>
+template <typename R, bool Dev, int C, int BD>
 __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
                                          int x, int y, int z, uint32_t s, R *__restrict__ fout)
 {
@@ -385,7 +385,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
         R du2 = (tp * F[c][2]) / P.amass[c];
         R b[19];
         if constexpr (Dev) {
-            boltz_dev(P.bdist, ux, uy, uz, du0, du1, du2, b);
+            boltz_dev(BD, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 // deviation of rho*b_i from w_i*rho0: w_i*drho + rho*(b_i - w_i)
@@ -395,7 +395,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
                          n[c][i] - P.omega[c] * (n[c][i] - deq));
             }
         } else {
-            boltz_dist<R>(P.bdist, ux, uy, uz, du0, du1, du2, b);
+            boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];
@@ -409,7 +409,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
 // --------------------------------------------------------------------------
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, int BD>
 __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ aux,
@@ -426,10 +426,10 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
         for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
     R F[C][3];
     sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
-    bgk_site<R, Dev, C>(P, n, F, x, y, z, s, fout);
+    bgk_site<R, Dev, C, BD>(P, n, F, x, y, z, s, fout);
 }
 
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, int BD>
 __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ aux,
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
 {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    collide_site<R, Dev, C, LE>(P, f, face_pc, aux, face_aux, db, fout, s);
+    collide_site<R, Dev, C, LE, BD>(P, f, face_pc, aux, face_aux, db, fout, s);
 }
 
 // --------------------------------------------------------------------------
@@ -518,13 +518,18 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     // psi of plane zp for the tile and halo into ring slot; the owner's
     // streamed populations of that plane go to its stage column
     auto fill = [&](int zp, int slot) __attribute__((always_inline)) {
+        // opaque copies of the lane coordinates: keeps the compiler from
+        // hoisting the 19 per-direction offsets out of the z loop (they
+        // would stay live in VGPRs across the collide)
+        int xo = x, yo = y;
+        asm volatile("" : "+v"(xo), "+v"(yo));
         if (own) {
             int zs = wrap1(zp, P.nz);
             R r0 = R(0), r1 = R(0);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
-                R v0 = pull<R, 2, LE>(f, face_pc, P, 0, i, x, y, zs);
-                R v1 = pull<R, 2, LE>(f, face_pc, P, 1, i, x, y, zs);
+                R v0 = pull<R, 2, LE>(f, face_pc, P, 0, i, xo, yo, zs);
+                R v1 = pull<R, 2, LE>(f, face_pc, P, 1, i, xo, yo, zs);
                 if constexpr (Stage) {
                     stage[i][tid] = v0;
                     stage[19 + i][tid] = v1;
@@ -547,7 +552,9 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
         for (int r = 0; r < (NH > T ? 2 : 1); r++)
             if (hv[r]) {
                 R p0, p1;
-                psi_at<R, Dev, LE>(P, f, face_pc, face_aux, hgx[r], hgy[r], zp, p0, p1);
+                int gx = hgx[r], gy = hgy[r];
+                asm volatile("" : "+v"(gx), "+v"(gy));
+                psi_at<R, Dev, LE>(P, f, face_pc, face_aux, gx, gy, zp, p0, p1);
                 ring[slot][0][hq[r]] = p0;
                 ring[slot][1][hq[r]] = p1;
             }
@@ -597,7 +604,7 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             F[0][0] = (-(P.g_br * frb0)) * psr;
             F[0][1] = (-(P.g_br * frb1)) * psr;
             F[0][2] = (-(P.g_br * frb2)) * psr;
-            bgk_site<R, Dev, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+            bgk_site<R, Dev, 2, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
         }
         __syncthreads();
         int t = sm;
@@ -846,7 +853,7 @@ __global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restr
     }
 }
 
-template <typename R, bool Dev, int C>
+template <typename R, bool Dev, int C, int BD>
 __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__restrict__ lat,
                                                        const uint2 *__restrict__ items,
                                                        uint32_t nitems, const R *__restrict__ f,
@@ -859,8 +866,9 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
         if (s < L.P.ns)
-            collide_site<R, Dev, C, false>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
-                                           C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
+            if (L.P.bdist == BD)
+                collide_site<R, Dev, C, false, BD>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
+                                                   C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
     }
 }
 

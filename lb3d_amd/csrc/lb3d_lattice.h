@@ -67,8 +67,12 @@ __device__ __forceinline__ void boltz_dist(int bdist, R ux, R uy, R uz, R dux, R
         R uholdit = uke * holdit;
 #pragma unroll
         for (int s = 0; s < 18; s++) {
-            R cdotu = R(CX[s]) * ux + R(CY[s]) * uy + R(CZ[s]) * uz;
-            R cdotdu = R(CX[s]) * dux + R(CY[s]) * duy + R(CZ[s]) * duz;
+            // c.u over the nonzero components of c (a zero term only changes the
+            // sign of an exact zero)
+            R cdotu = R(0), cdotdu = R(0);
+            if (CX[s] != 0) { cdotu = cdotu + R(CX[s]) * ux; cdotdu = cdotdu + R(CX[s]) * dux; }
+            if (CY[s] != 0) { cdotu = cdotu + R(CY[s]) * uy; cdotdu = cdotdu + R(CY[s]) * duy; }
+            if (CZ[s] != 0) { cdotu = cdotu + R(CZ[s]) * uz; cdotdu = cdotdu + R(CZ[s]) * duz; }
             R ch = cdotu * holdit;
             R cdh = cdotdu * holdit;
             R ch2 = ch * ch;
@@ -116,8 +120,12 @@ __device__ __forceinline__ void boltz_dev(int bdist, float ux, float uy, float u
         float uh = 3.0f * uke;
 #pragma unroll
         for (int s = 0; s < 18; s++) {
-            float ch = 3.0f * (float(CX[s]) * ux + float(CY[s]) * uy + float(CZ[s]) * uz);
-            float cdh = 3.0f * (float(CX[s]) * dux + float(CY[s]) * duy + float(CZ[s]) * duz);
+            float cu = 0.0f, cd = 0.0f;
+            if (CX[s] != 0) { cu = cu + float(CX[s]) * ux; cd = cd + float(CX[s]) * dux; }
+            if (CY[s] != 0) { cu = cu + float(CY[s]) * uy; cd = cd + float(CY[s]) * duy; }
+            if (CZ[s] != 0) { cu = cu + float(CZ[s]) * uz; cd = cd + float(CZ[s]) * duz; }
+            float ch = 3.0f * cu;
+            float cdh = 3.0f * cd;
             float ch2 = ch * ch;
             bw[s] = w * (ch + 0.5f * ch2 - uh + (ch * ch2) * (1.0f / 6.0f) - uh * ch + cdh +
                          ch * cdh - 3.0f * udotdu + 0.5f * ch2 * cdh - 3.0f * ch * udotdu - uh * cdh);

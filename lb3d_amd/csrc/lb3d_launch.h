@@ -176,19 +176,28 @@ class LauncherT : public Launcher {
     {
         if (C == 3) k_dipole<R, Dev><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db);
     }
-    void collide(const DevBufs &b, hipStream_t s) override
+    template <int BD>
+    void collide_bd(const DevBufs &b, hipStream_t s)
     {
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (C == 3)
-            k_collide<R, Dev, 3, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                  (const R *)b.db, fo);
+            k_collide<R, Dev, 3, false, BD><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                      (const R *)b.db, fo);
         else if (le)
-            k_collide<R, Dev, 2, true><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, (const R *)b.aux,
-                                                                 (const R *)b.face_aux, nullptr, fo);
+            k_collide<R, Dev, 2, true, BD><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc,
+                                                                     (const R *)b.aux, (const R *)b.face_aux,
+                                                                     nullptr, fo);
         else
-            k_collide<R, Dev, 2, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                  nullptr, fo);
+            k_collide<R, Dev, 2, false, BD><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                      nullptr, fo);
+    }
+    void collide(const DevBufs &b, hipStream_t s) override
+    {
+        if (P.bdist == 0)
+            collide_bd<0>(b, s);
+        else
+            collide_bd<2>(b, s);
     }
     void le_faces(const DevBufs &b, bool post, hipStream_t s) override
     {
@@ -234,7 +243,7 @@ class LauncherT : public Launcher {
     }
     bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, bool st, hipStream_t s, bool launch)
     {
-        if (C != 2 || lz < 1) return false;
+        if (C != 2 || lz < 1 || P.bdist != 2) return false;
         if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, st, s, launch);
         if (tx == 32 && ty == 8) return fused_t<32, 8>(b, lz, st, s, launch);
         if (tx == 64 && ty == 8) return fused_t<64, 8>(b, lz, st, s, launch);
@@ -282,6 +291,8 @@ class LauncherT : public Launcher {
 template <typename R, bool Dev>
 class BatchLauncherT : public BatchLauncher {
   public:
+    int bd_mask = 0;  // equilibrium forms present in the batch (bit 0: bdist 0, bit 2: bdist 2)
+    void set_bdist_mask(int m) override { bd_mask = m; }
     size_t lat_bytes() const override { return sizeof(BatchLat<R>); }
     void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                   uint64_t db_off) const override
@@ -306,12 +317,22 @@ class BatchLauncherT : public BatchLauncher {
         } else if (ph == 1) {
             if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db);
         } else {
-            if (ncomp == 3)
-                k_batch_collide<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                  (const R *)db, (R *)fout);
-            else
-                k_batch_collide<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                  nullptr, (R *)fout);
+            // every lattice's collide runs in the launch of its own equilibrium form
+            for (int bd : {0, 2}) {
+                if (!(bd_mask & (1 << bd))) continue;
+                if (ncomp == 3 && bd == 0)
+                    k_batch_collide<R, Dev, 3, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                         (const R *)db, (R *)fout);
+                else if (ncomp == 3)
+                    k_batch_collide<R, Dev, 3, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                         (const R *)db, (R *)fout);
+                else if (bd == 0)
+                    k_batch_collide<R, Dev, 2, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                         nullptr, (R *)fout);
+                else
+                    k_batch_collide<R, Dev, 2, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                         nullptr, (R *)fout);
+            }
         }
     }
     void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,

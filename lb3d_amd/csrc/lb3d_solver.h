@@ -54,6 +54,7 @@ class BatchLauncher {
   public:
     virtual ~BatchLauncher() {}
     virtual size_t lat_bytes() const = 0;   // sizeof(BatchLat<R>)
+    virtual void set_bdist_mask(int mask) = 0;
     // host image of the lattice table entry k
     virtual void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                           uint64_t db_off) const = 0;

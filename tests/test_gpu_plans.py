@@ -48,6 +48,13 @@ def test_fused_equals_two_pass_bitwise(precision, dims, le):
                 f"{dims} le={le} mode {mode} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
 
 
+def test_one_pass_plan_rejected_where_it_does_not_exist():
+    for kw in (dict(ncomp=3), dict(ncomp=2, bdist=0)):
+        g = Lattice(8, 8, 8, **kw)
+        with pytest.raises(LB3DError):
+            g.set_plan(1, 64, 4, 8)
+
+
 def test_fused_fp64_matches_oracle_32cubed_100_steps():
     o, g = pair(32, 32, 32, ncomp=2, g_br=0.08)
     g.set_plan(1, 32, 4, 16)
