@@ -36,11 +36,12 @@ def bytes_per_site(ncomp, ternary, real_bytes):
     return 2 * (19 * ncomp + (3 if ternary else 0)) * real_bytes
 
 
-def run_form(n, ncomp, precision, steps, warmup, profile_steps):
+def run_form(n, ncomp, precision, steps, warmup, profile_steps, rho_shift=0.0):
     from lb3d_amd.gpu import Lattice
 
     g = Lattice(n, n, n, ncomp=ncomp, precision=precision, g_br=0.08, bdist=2)
-    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
+    m = 0.7 + rho_shift
+    g.init_noise(1, rho_mean=(m, m, m), amp=(0.01, 0.01, 0.0), dipoles=False)
     g.bc_init()
     report = g.autotune(trial_steps=3)
     chosen = report.strip().splitlines()[-1]
@@ -139,6 +140,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--ensemble-fp64-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-floor", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -168,12 +170,21 @@ def main():
     r32 = run_form(n, ncomp, LB3D_FP32_DEV, args.steps, args.warmup, args.profile_steps)
     print(f"[bench] FP32 {r32['mlups']:.1f} MLUPS {r32['kernels']}", file=sys.stderr, flush=True)
 
-    d = np.abs(r32["rho"] - r64["rho"])
-    ref = np.abs(r64["rho"])
-    acc = dict(steps=args.warmup + args.steps,
-               linf_rel=float(d.max() / ref.max()),
-               l2_rel=float(np.sqrt((d ** 2).sum() / (ref ** 2).sum())),
+    def errs(a, b):
+        d = np.abs(a - b)
+        ref = np.abs(b)
+        return float(d.max() / ref.max()), float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))
+
+    linf, l2 = errs(r32["rho"], r64["rho"])
+    acc = dict(steps=args.warmup + args.steps, linf_rel=linf, l2_rel=l2,
                mass_rel=float(max(abs(a - b) / abs(b) for a, b in zip(r32["mass"][:ncomp], r64["mass"][:ncomp]))))
+    if not args.no_floor:
+        # intrinsic floor of the unstable flow: FP64 against FP64 whose initial
+        # mean density differs by 2^-51 (about 2 ulp)
+        print("[bench] FP64 floor run (perturbed initial state)", file=sys.stderr, flush=True)
+        rf = run_form(n, ncomp, LB3D_FP64, args.steps, args.warmup, 1, rho_shift=2.0 ** -51)
+        flinf, fl2 = errs(rf["rho"], r64["rho"])
+        acc["fp64_perturbed_floor"] = dict(linf_rel=flinf, l2_rel=fl2, perturbation="initial mean density + 2^-51")
 
     ns = n ** 3
     B32 = bytes_per_site(ncomp, False, 4)
