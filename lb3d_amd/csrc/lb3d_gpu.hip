@@ -200,7 +200,7 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc db: ") + hipGetErrorString(e));
     if (c->le) {
         if ((e = hipMalloc(&c->b.face_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
-            (e = hipMalloc(&c->b.face_aux, 2 * 2 * plane * rs)) != hipSuccess)
+            (e = hipMalloc(&c->b.face_aux, 2 * 4 * plane * rs)) != hipSuccess)
             return cleanup(LB3D_ENOMEM, std::string("hipMalloc LE planes: ") + hipGetErrorString(e));
     }
     if ((e = hipMalloc((void **)&c->b.le, sizeof(LEState))) != hipSuccess ||
@@ -368,7 +368,7 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
                         m[2] = m[2] + ng * (double)host::CZ[i];
                     }
                     if (c->aux_valid) {
-                        double r = aux[(size_t)k * ns + s];
+                        double r = aux[(size_t)(C + k) * ns + s];
                         rho = dev ? rho0_of(c, k) + r : r;
                     }
                     S[24 * k + 19] = rho;
@@ -376,7 +376,7 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
                 }
                 if (C == 3)
                     for (int a = 0; a < 3; a++) {
-                        S[72 + a] = c->aux_valid ? aux[(size_t)(3 + a) * ns + s] : 0.0;
+                        S[72 + a] = c->aux_valid ? aux[(size_t)(6 + a) * ns + s] : 0.0;
                         S[75 + a] = db[(size_t)a * ns + s];
                     }
                 S[rock_off] = 0.0;

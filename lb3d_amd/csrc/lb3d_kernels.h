@@ -4,14 +4,13 @@
 // index wraps):
 //   f[(c*19 + i)*ns + s]   populations, two buffers (pull: read one, write other)
 //   aux[k*ns + s]          moments of the streamed state, written by k_moments:
-//                          binary  k = rho_r, rho_b
-//                          ternary k = rho_r, rho_b, rho_s, da_x, da_y, da_z
-//                          (psi = 1 - exp(-rho) is recomputed where it is used)
+//                          binary  k = psi_r, psi_b, rho_r, rho_b
+//                          ternary k = psi_r, psi_b, psi_s, rho_r, rho_b, rho_s, da_x, da_y, da_z
 //   db[k*ns + s]           ternary dipole field
 //   face_pc[((f*C + c)*19 + i)*ny*nz + z*ny + y]  Lees-Edwards x-halo planes
 //                          (f = 0: x=0 plane, f = 1: x=nx+1 plane) of the
 //                          post-collision populations, built by k_le_faces<false>
-//   face_aux[(f*2 + c)*ny*nz + z*ny + y]  densities of the post-advection
+//   face_aux[(f*4 + k)*ny*nz + z*ny + y]  moments of the post-advection
 //                          x-halo planes (binary), built by k_le_faces<true>
 //
 // Form FP64 (Dev = false, R = double): the reference's values and arithmetic
@@ -24,8 +23,8 @@
 
 namespace lb3d {
 
-constexpr int NAUX2 = 2;
-constexpr int NAUX3 = 6;
+constexpr int NAUX2 = 4;
+constexpr int NAUX3 = 9;
 
 __device__ __forceinline__ void site_xyz(uint32_t s, int nx, int ny, int &x, int &y, int &z)
 {
@@ -91,7 +90,7 @@ __device__ __forceinline__ R aux_at(const R *__restrict__ aux, const R *__restri
         if (x < 0 || x >= P.nx) {
             int fc = x < 0 ? 0 : 1;
             uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
-            return ld_plane(plane_of(face_aux, (uint32_t)(fc * 2 + k), plane),
+            return ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + k), plane),
                             (uint32_t)zs * (uint32_t)P.ny + (uint32_t)ys);
         }
     } else {
@@ -150,11 +149,14 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
         }
     }
 #pragma unroll
-    for (int c = 0; c < C; c++) st_plane(aux + (size_t)c * P.ns, s, rho[c]);
+    for (int c = 0; c < C; c++) {
+        st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
+        st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
+    }
     if constexpr (C == 3) {
-        st_plane(aux + (size_t)3 * P.ns, s, da0);
-        st_plane(aux + (size_t)4 * P.ns, s, da1);
-        st_plane(aux + (size_t)5 * P.ns, s, da2);
+        st_plane(aux + (size_t)6 * P.ns, s, da0);
+        st_plane(aux + (size_t)7 * P.ns, s, da1);
+        st_plane(aux + (size_t)8 * P.ns, s, da2);
     }
 }
 
@@ -176,9 +178,9 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
-    R hs0 = ld_plane(plane_of(aux, 3u, P.ns), s), hs1 = ld_plane(plane_of(aux, 4u, P.ns), s),
-      hs2 = ld_plane(plane_of(aux, 5u, P.ns), s);
-    R rho_s = ld_plane(plane_of(aux, 2u, P.ns), s);
+    R hs0 = ld_plane(plane_of(aux, 6u, P.ns), s), hs1 = ld_plane(plane_of(aux, 7u, P.ns), s),
+      hs2 = ld_plane(plane_of(aux, 8u, P.ns), s);
+    R rho_s = ld_plane(plane_of(aux, 5u, P.ns), s);
     if constexpr (Dev) rho_s = P.rho0[2] + rho_s;
     R sN = rho_s > R(CLAMP_SURF) ? rho_s : R(CLAMP_SURF);
     R sd0 = hs0 / sN, sd1 = hs1 / sN, sd2 = hs2 / sN;
@@ -187,16 +189,16 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
     for (int i = 0; i < 18; i++) {
         int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
         // -sign(1,g_bs)*(rho_b - rho_r)(x+c_i)*g_i (raw densities, :361-369)
-        R rr = aux_at<R, false>(aux, nullptr, P, 0, xa, ya, za);
-        R rb = aux_at<R, false>(aux, nullptr, P, 1, xa, ya, za);
+        R rr = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
+        R rb = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
         R gs = R(GD[i]);
         R hc_m = -((P.sign_g_bs * (rb - rr)) * gs);
         if (CX[i] != 0) hc0 = hc0 + hc_m * R(CX[i]);
         if (CY[i] != 0) hc1 = hc1 + hc_m * R(CY[i]);
         if (CZ[i] != 0) hc2 = hc2 + hc_m * R(CZ[i]);
-        R p0 = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
-        R p1 = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
-        R p2 = aux_at<R, false>(aux, nullptr, P, 5, xa, ya, za);
+        R p0 = aux_at<R, false>(aux, nullptr, P, 6, xa, ya, za);
+        R p1 = aux_at<R, false>(aux, nullptr, P, 7, xa, ya, za);
+        R p2 = aux_at<R, false>(aux, nullptr, P, 8, xa, ya, za);
         R hs_dot = R(2.0) * (R(CX[i]) * p0 + R(CY[i]) * p1 + R(CZ[i]) * p2);
         hs0 = hs0 + (p0 - R(CX[i]) * hs_dot) * gs;
         hs1 = hs1 + (p1 - R(CY[i]) * hs_dot) * gs;
@@ -243,16 +245,15 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
 #pragma unroll
         for (int i = 0; i < 18; i++) {
             int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
-            R pr = psi_of<Dev>(aux_at<R, LE>(aux, face_aux, P, 0, xa, ya, za), P.emr0[0]);
-            R pb = psi_of<Dev>(aux_at<R, LE>(aux, face_aux, P, 1, xa, ya, za), P.emr0[1]);
+            R pr = aux_at<R, LE>(aux, face_aux, P, 0, xa, ya, za);
+            R pb = aux_at<R, LE>(aux, face_aux, P, 1, xa, ya, za);
             R green_r = pb * R(GD[i]);
             R green_b = pr * R(GD[i]);
             if (CX[i] != 0) { frb[0] = frb[0] + green_r * R(CX[i]); fbr[0] = fbr[0] + green_b * R(CX[i]); }
             if (CY[i] != 0) { frb[1] = frb[1] + green_r * R(CY[i]); fbr[1] = fbr[1] + green_b * R(CY[i]); }
             if (CZ[i] != 0) { frb[2] = frb[2] + green_r * R(CZ[i]); fbr[2] = fbr[2] + green_b * R(CZ[i]); }
         }
-        R psr = psi_of<Dev>(ld_plane(aux, s), P.emr0[0]);
-        R psb = psi_of<Dev>(ld_plane(plane_of(aux, 1u, P.ns), s), P.emr0[1]);
+        R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s);
         if constexpr (Dev) {
             psr = P.psi0[0] + psr;
             psb = P.psi0[1] + psb;
@@ -272,9 +273,9 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
             const R c0 = R(CX[i]), c1 = R(CY[i]), c2 = R(CZ[i]);
             const R gs = R(GD[i]);
-            R pr = psi_of<Dev>(aux_at<R, false>(aux, nullptr, P, 0, xa, ya, za), P.emr0[0]);
-            R pb = psi_of<Dev>(aux_at<R, false>(aux, nullptr, P, 1, xa, ya, za), P.emr0[1]);
-            R ps = psi_of<Dev>(aux_at<R, false>(aux, nullptr, P, 2, xa, ya, za), P.emr0[2]);
+            R pr = aux_at<R, false>(aux, nullptr, P, 0, xa, ya, za);
+            R pb = aux_at<R, false>(aux, nullptr, P, 1, xa, ya, za);
+            R ps = aux_at<R, false>(aux, nullptr, P, 2, xa, ya, za);
             if constexpr (Dev) ps = P.psi0[2] + ps;
             R green_r = pb * gs, green_b = pr * gs;
             if (CX[i] != 0) { frb[0] = frb[0] + green_r * c0; fbr[0] = fbr[0] + green_b * c0; }
@@ -299,9 +300,8 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
             fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
         }
-        R psr = psi_of<Dev>(ld_plane(aux, s), P.emr0[0]);
-        R psb = psi_of<Dev>(ld_plane(plane_of(aux, 1u, P.ns), s), P.emr0[1]);
-        R pss = psi_of<Dev>(ld_plane(plane_of(aux, 2u, P.ns), s), P.emr0[2]);
+        R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s),
+          pss = ld_plane(plane_of(aux, 2u, P.ns), s);
         if constexpr (Dev) {
             psr = P.psi0[0] + psr;
             psb = P.psi0[1] + psb;
@@ -461,8 +461,8 @@ __device__ __forceinline__ void psi_at(const Phys<R> &P, const R *__restrict__ f
         uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
         int fc = x < 0 ? 0 : 1;
         uint32_t q = (uint32_t)wrap1(z, P.nz) * (uint32_t)P.ny + (uint32_t)wrap1(y, P.ny);
-        p0 = psi_of<Dev>(ld_plane(plane_of(face_aux, (uint32_t)(fc * 2 + 0), plane), q), P.emr0[0]);
-        p1 = psi_of<Dev>(ld_plane(plane_of(face_aux, (uint32_t)(fc * 2 + 1), plane), q), P.emr0[1]);
+        p0 = ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + 0), plane), q);
+        p1 = ld_plane(plane_of(face_aux, (uint32_t)(fc * 4 + 1), plane), q);
         return;
     }
     int xs = wrap1(x, P.nx), ys = wrap1(y, P.ny), zs = wrap1(z, P.nz);
@@ -793,7 +793,8 @@ __global__ __launch_bounds__(128) void k_le_faces(Phys<R> P, const LEState *__re
                 R rho = R(0);
 #pragma unroll
                 for (int i = 0; i < 19; i++) rho = rho + (fc == 0 ? lo[c][i] : hi[c][i]) * R(GD[i]);
-                face_aux[(uint32_t)(fc * 2 + c) * plane + t] = rho;
+                face_aux[(uint32_t)(fc * 4 + c) * plane + t] = psi_of<Dev>(rho, P.emr0[c]);
+                face_aux[(uint32_t)(fc * 4 + 2 + c) * plane + t] = rho;
             }
         }
     }
