@@ -131,14 +131,18 @@ int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
 
 /* Autotuner (not a reference routine): times every execution plan of this
  * lattice -- the two-pass kernels and, for binary lattices, the one-pass
- * z-march kernel over its tile shapes and z-chunks -- on trial_steps trial
- * steps each, keeps the fastest and writes a text report (may be NULL).  The
- * lattice state is unchanged. */
+ * z-march kernel over its tile shapes and z-chunks, then the XCD chunking of
+ * the best -- on trial_steps trial steps each, keeps the fastest and writes a
+ * text report (may be NULL).  The lattice state is unchanged. */
 int lb3d_gpu_autotune(lb3d_ctx *ctx, int32_t trial_steps, char *report, int32_t report_len);
 /* Choose the execution plan directly: fused = 0 (two-pass), 1 (one-pass
  * kernel with tile tx x ty and z-chunk lz, look-ahead plane staged in LDS) or
- * 2 (the same with the look-ahead plane in registers); 1 and 2 binary only. */
-int lb3d_gpu_set_plan(lb3d_ctx *ctx, int32_t fused, int32_t tx, int32_t ty, int32_t lz);
+ * 2 (the same with the look-ahead plane in registers); 1 and 2 binary only.
+ * xcd_group: tiles that one XCD (and its L2) takes in a row; 1 is the plain
+ * round-robin order, 0 one contiguous run of tiles per XCD.  Results do not
+ * depend on the plan. */
+int lb3d_gpu_set_plan(lb3d_ctx *ctx, int32_t fused, int32_t tx, int32_t ty, int32_t lz,
+                      int32_t xcd_group);
 
 /* Current step counter nt. */
 int lb3d_gpu_get_nt(lb3d_ctx *ctx, int32_t *nt);
@@ -197,9 +201,12 @@ int lb3d_gpu_batch_get_populations(lb3d_batch *b, int32_t k, double *out);
 /* Sanity reductions of lattice k (as lb3d_gpu_diagnostics). */
 int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double momentum[3],
                                int64_t counts[3]);
-/* Plan: persistent launches with nwg workgroups (sequential = 0), or one
- * lattice after another with the single-lattice kernels (sequential = 1). */
-int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg);
+/* Plan: persistent launches with nwg workgroups (sequential = 0; nwg is capped
+ * at the tile count and rounded up to a multiple of 8, one per XCD) taking
+ * the work list in chunks of xcd_group items per XCD (as lb3d_gpu_set_plan),
+ * or one lattice after another with the single-lattice kernels
+ * (sequential = 1). */
+int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int32_t xcd_group);
 /* Autotuner: times the sequential plan and persistent grid sizes on the
  * batch's own state (unchanged) and keeps the fastest. */
 int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, int32_t report_len);

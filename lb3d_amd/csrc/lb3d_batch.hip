@@ -38,7 +38,8 @@ struct lb3d_batch {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int cur = 0, nt = 0;
-    unsigned nwg = 2048;     // persistent grid size
+    unsigned nwg = 2048;     // persistent grid size, a multiple of 8
+    uint32_t xg = 0;         // work items per XCD chunk (xcd_tile)
     bool sequential = false; // one lattice after another with the single-lattice kernels
 };
 
@@ -182,7 +183,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
     if ((e = hipMemcpy(b->lat, lat.data(), lat.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->items, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_destroy("hipMemcpy", e);
-    b->nwg = b->nitems < 2048 ? b->nitems : 2048;
+    b->nwg = xcd_round(b->nitems < 2048 ? b->nitems : 2048);
     *out = b;
     return LB3D_OK;
 }
@@ -291,13 +292,16 @@ int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double 
     return LB3D_OK;
 }
 
-int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg)
+int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int32_t xcd_group)
 {
     clear_error();
     if (!b) return set_error(LB3D_EINVAL, "NULL batch");
     if (!sequential && nwg < 1) return set_error(LB3D_EINVAL, "nwg must be >= 1");
+    if (xcd_group < 0) return set_error(LB3D_EINVAL, "xcd_group must be >= 0");
     b->sequential = sequential != 0;
-    if (!b->sequential) b->nwg = (unsigned)nwg < b->nitems ? (unsigned)nwg : b->nitems;
+    b->xg = (uint32_t)xcd_group;
+    b->B->xg = b->xg;
+    if (!b->sequential) b->nwg = xcd_round((unsigned)nwg < b->nitems ? (unsigned)nwg : b->nitems);
     return drop_graph(b);
 }
 
@@ -312,11 +316,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     struct Cand {
         bool seq;
         unsigned nwg;
+        uint32_t xg;
     };
-    std::vector<Cand> cands = {{true, 0}};
+    std::vector<Cand> cands = {{true, 0, 0}};
     for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
-        if (g < b->nitems) cands.push_back({false, g});
-    cands.push_back({false, b->nitems});
+        if (g < b->nitems) cands.push_back({false, g, 0});
+    cands.push_back({false, xcd_round(b->nitems), 0});
     void *db_save = nullptr;
     size_t db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
     if (db_bytes) {
@@ -326,14 +331,13 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     hipEvent_t e0, e1;
     BTRY(hipEventCreate(&e0));
     BTRY(hipEventCreate(&e1));
-    bool seq0 = b->sequential;
-    unsigned nwg0 = b->nwg;
     double best = 1e300;
-    Cand bc = {seq0, nwg0};
+    Cand bc = {b->sequential, b->nwg, b->xg};
     std::string rep;
-    for (const Cand &c : cands) {
+    auto time_cand = [&](const Cand &c) -> int {
         b->sequential = c.seq;
         b->nwg = c.nwg;
+        b->B->xg = c.xg;
         auto trial = [&]() {
             enqueue_batch_step(b);
             b->cur = 1 - b->cur;  // undo the swap: the state stays in f[cur]
@@ -348,12 +352,21 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         BTRY(hipEventElapsedTime(&ms, e0, e1));
         double per = ms / trial_steps;
         char buf[96];
-        std::snprintf(buf, sizeof buf, "%s nwg=%u %.4f ms\n", c.seq ? "sequential" : "persistent", c.nwg, per);
+        std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u %.4f ms\n", c.seq ? "sequential" : "persistent", c.nwg,
+                      c.xg, per);
         rep += buf;
         if (per < best) {
             best = per;
             bc = c;
         }
+        return LB3D_OK;
+    };
+    for (const Cand &c : cands)
+        if (int rc = time_cand(c)) return rc;
+    if (!bc.seq) {  // the XCD chunking of the best persistent grid
+        const Cand top = bc;
+        for (uint32_t xg : {1u, 4u, 16u, 64u, 256u})
+            if (int rc = time_cand({false, top.nwg, xg})) return rc;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -364,9 +377,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     }
     b->sequential = bc.seq;
     b->nwg = bc.nwg;
+    b->xg = bc.xg;
+    b->B->xg = bc.xg;
     drop_graph(b);
     char buf[96];
-    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u %.4f ms\n", bc.seq ? "sequential" : "persistent", bc.nwg, best);
+    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u %.4f ms\n", bc.seq ? "sequential" : "persistent",
+                  bc.nwg, bc.xg, best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);

@@ -45,6 +45,7 @@ struct Plan {
     bool fused = false;
     int tx = 64, ty = 4, lz = 0;
     bool stage = true;  // look-ahead populations in LDS (true) or registers
+    uint32_t xg = 1;    // tiles per XCD chunk (xcd_tile): 1 plain order, 0 one run per XCD
 };
 
 }  // namespace
@@ -213,7 +214,9 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
         c->plan.tx = c->L->fused_supported(64, 4, true) ? 64 : 32;
         c->plan.ty = 4;
         c->plan.lz = p->nz < 64 ? p->nz : 64;
+        c->plan.xg = 0;
     }
+    c->L->xg = c->plan.xg;
     c->host_le.shear_u = p->shear_u;
     c->host_le.shear_tmp = p->shear_u;
     c->host_le.shear_sum = p->shear_sum;
@@ -615,6 +618,7 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
                     q.ty = FUSED_SHAPES[k][1];
                     q.lz = lz < c->p.nz ? lz : c->p.nz;
                     q.stage = st == 0;
+                    q.xg = 0;
                     cands.push_back(q);
                 }
     }
@@ -630,8 +634,9 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     double best = 1e300;
     Plan bestp = saved;
     std::string rep;
-    for (const Plan &q : cands) {
+    auto time_plan = [&](const Plan &q, double &per) -> int {
         c->plan = q;
+        c->L->xg = q.xg;
         auto trial = [&]() {
             if (c->le) c->L->le_faces(c->b, false, c->stream);
             if (!q.fused) c->L->moments(c->b, c->stream);
@@ -646,16 +651,28 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        double per = ms / trial_steps;
-        char buf[96];
-        std::snprintf(buf, sizeof buf, "%s %dx%d lz=%d %s %.4f ms\n", q.fused ? "fused" : "two-pass",
+        per = ms / trial_steps;
+        char buf[112];
+        std::snprintf(buf, sizeof buf, "%s %dx%d lz=%d %s xg=%u %.4f ms\n", q.fused ? "fused" : "two-pass",
                       q.fused ? q.tx : 0, q.fused ? q.ty : 0, q.fused ? q.lz : 0,
-                      q.fused ? (q.stage ? "lds" : "regs") : "-", per);
+                      q.fused ? (q.stage ? "lds" : "regs") : "-", q.xg, per);
         rep += buf;
         if (per < best) {
             best = per;
             bestp = q;
         }
+        return LB3D_OK;
+    };
+    double per = 0.0;
+    for (const Plan &q : cands)
+        if (int rc = time_plan(q, per)) return rc;
+    // then the XCD chunking of the best plan
+    const Plan top = bestp;
+    for (uint32_t xg : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 0u}) {
+        if (xg == top.xg) continue;
+        Plan q = top;
+        q.xg = xg;
+        if (int rc = time_plan(q, per)) return rc;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -665,16 +682,17 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         (void)hipFree(db_save);
     }
     c->plan = bestp;
+    c->L->xg = bestp.xg;
     c->tuned = true;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
     // aux (ternary two-pass) was overwritten by the trials with values of the same state
-    char buf[96];
-    std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %s %.4f ms\n", bestp.fused ? "fused" : "two-pass",
+    char buf[112];
+    std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %s xg=%u %.4f ms\n", bestp.fused ? "fused" : "two-pass",
                   bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0,
-                  bestp.fused ? (bestp.stage ? "lds" : "regs") : "-", best);
+                  bestp.fused ? (bestp.stage ? "lds" : "regs") : "-", bestp.xg, best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
@@ -683,11 +701,13 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     return LB3D_OK;
 }
 
-int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_t lz)
+int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_t lz, int32_t xcd_group)
 {
     g_err.clear();
     if (!c) return fail(LB3D_EINVAL, "NULL context");
+    if (xcd_group < 0) return fail(LB3D_EINVAL, "xcd_group must be >= 0");
     Plan q;
+    q.xg = (uint32_t)xcd_group;
     q.fused = fused != 0;
     q.stage = fused != 2;  // fused = 2: look-ahead in registers
     if (q.fused) {
@@ -700,6 +720,7 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
         q.lz = lz < c->p.nz ? lz : c->p.nz;
     }
     c->plan = q;
+    c->L->xg = q.xg;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;

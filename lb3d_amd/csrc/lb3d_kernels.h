@@ -26,6 +26,24 @@ namespace lb3d {
 constexpr int NAUX2 = 4;
 constexpr int NAUX3 = 9;
 
+// Work-groups are dispatched round-robin over the chip's 8 XCDs (XCD =
+// work-group index mod 8), each XCD with its own L2.  xcd_tile maps the
+// work-group index b of a grid of n8 (a multiple of 8) to a tile so that each
+// XCD runs chunks of xg consecutive tiles: the neighbour rows and planes that
+// the pulls and stencils of a tile re-read are then fetched by the same L2.
+// xg = 1 is the plain order; xg = 0 (or >= n8/8) gives each XCD one
+// contiguous run.  A bijection on [0, n8); tiles >= the tile count idle.
+constexpr uint32_t NXCD = 8;
+__host__ __device__ constexpr uint32_t xcd_round(uint32_t n) { return (n + NXCD - 1) / NXCD * NXCD; }
+__host__ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n8, uint32_t xg)
+{
+    uint32_t per = n8 / NXCD, j = b / NXCD, k = b % NXCD;
+    if (xg == 0 || xg > per) xg = per;
+    uint32_t c = j / xg, o = j - c * xg;
+    uint32_t w = per - c * xg < xg ? per - c * xg : xg;  // the last chunk may be narrower
+    return c * NXCD * xg + k * w + o;
+}
+
 __device__ __forceinline__ void site_xyz(uint32_t s, int nx, int ny, int &x, int &y, int &z)
 {
     uint32_t t = s / (uint32_t)nx;
@@ -163,9 +181,10 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
 template <typename R, bool Dev, int C, bool LE>
 __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
-                                                 const R *__restrict__ db, R *__restrict__ aux)
+                                                 const R *__restrict__ db, R *__restrict__ aux,
+                                                 uint32_t xg)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     moments_site<R, Dev, C, LE>(P, f, face_pc, db, aux, s);
 }
@@ -224,9 +243,9 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
 
 template <typename R, bool Dev>
 __global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__ aux,
-                                                R *__restrict__ db)
+                                                R *__restrict__ db, uint32_t xg)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     dipole_site<R, Dev>(P, aux, db, s);
 }
@@ -434,9 +453,10 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ aux,
                                                  const R *__restrict__ face_aux,
-                                                 const R *__restrict__ db, R *__restrict__ fout)
+                                                 const R *__restrict__ db, R *__restrict__ fout,
+                                                 uint32_t xg)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     collide_site<R, Dev, C, LE, BD>(P, f, face_pc, aux, face_aux, db, fout, s);
 }
@@ -480,7 +500,7 @@ template <typename R, bool Dev, bool LE, int TX, int TY, bool Stage>
 __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restrict__ f,
                                                     const R *__restrict__ face_pc,
                                                     const R *__restrict__ face_aux,
-                                                    R *__restrict__ fout, int lz)
+                                                    R *__restrict__ fout, int lz, uint32_t xg)
 {
     constexpr int T = TX * TY, LX = TX + 2, LY = TY + 2, LP = LX * LY;
     constexpr int NH = 2 * LX + 2 * TY;  // halo positions per plane
@@ -490,7 +510,12 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     R nxt[2][19];
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = tid / TX;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * lz;
+    // tile (bx, by, bz) of a gx x gy x gz grid, x fastest, placed by xcd_tile
+    const uint32_t gx = ((uint32_t)P.nx + TX - 1) / TX, gy = ((uint32_t)P.ny + TY - 1) / TY;
+    const uint32_t gz = ((uint32_t)P.nz + lz - 1) / (uint32_t)lz;
+    const uint32_t tl = xcd_tile(blockIdx.x, gridDim.x, xg);
+    if (tl >= gx * gy * gz) return;  // uniform over the block
+    const int x0 = (int)(tl % gx) * TX, y0 = (int)((tl / gx) % gy) * TY, z0 = (int)(tl / (gx * gy)) * lz;
     const int x = x0 + lx, y = y0 + ly;
     const bool own = x < P.nx && y < P.ny;
     const int zend = min(z0 + lz, P.nz);
@@ -807,7 +832,9 @@ __global__ __launch_bounds__(128) void k_le_faces(Phys<R> P, const LEState *__re
 // Batched ensemble: many independent periodic lattices of uneven sizes in the
 // same launches.  Each lattice has its own parameters and a slice of the
 // shared buffers; the work list holds (lattice, first site) tiles of 256
-// sites.  Persistent grid: a fixed number of workgroups strides over the list.
+// sites.  Persistent grid: a fixed number of workgroups (a multiple of 8)
+// strides over the list in xcd_tile order, which keeps each XCD on its own
+// chunks of the list.
 template <typename R>
 struct BatchLat {
     Phys<R> P;
@@ -827,9 +854,12 @@ __global__ __launch_bounds__(256) void k_batch_moments(const BatchLat<R> *__rest
                                                        const uint2 *__restrict__ items,
                                                        uint32_t nitems, const R *__restrict__ f,
                                                        const R *__restrict__ db,
-                                                       R *__restrict__ aux)
+                                                       R *__restrict__ aux, uint32_t xg)
 {
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const uint32_t n8 = xcd_round(nitems);
+    for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
+        uint32_t it = xcd_tile(t, n8, xg);
+        if (it >= nitems) continue;
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
@@ -843,9 +873,12 @@ template <typename R, bool Dev>
 __global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restrict__ lat,
                                                       const uint2 *__restrict__ items,
                                                       uint32_t nitems, const R *__restrict__ aux,
-                                                      R *__restrict__ db)
+                                                      R *__restrict__ db, uint32_t xg)
 {
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const uint32_t n8 = xcd_round(nitems);
+    for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
+        uint32_t it = xcd_tile(t, n8, xg);
+        if (it >= nitems) continue;
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
@@ -859,9 +892,12 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
                                                        uint32_t nitems, const R *__restrict__ f,
                                                        const R *__restrict__ aux,
                                                        const R *__restrict__ db,
-                                                       R *__restrict__ fout)
+                                                       R *__restrict__ fout, uint32_t xg)
 {
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const uint32_t n8 = xcd_round(nitems);
+    for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
+        uint32_t it = xcd_tile(t, n8, xg);
+        if (it >= nitems) continue;
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;

@@ -159,6 +159,8 @@ class LauncherT : public Launcher {
     {
     }
     unsigned blocks() const { return (P.ns + BLOCK - 1) / BLOCK; }
+    // grid of the stepping kernels: whole XCD rounds (xcd_tile order)
+    unsigned tiles() const { return xcd_round(blocks()); }
     unsigned diag_blocks() const override { return blocks(); }
     size_t real_bytes() const override { return sizeof(R); }
 
@@ -166,15 +168,15 @@ class LauncherT : public Launcher {
     {
         const R *f = (const R *)b.f[b.cur];
         if (C == 3)
-            k_moments<R, Dev, 3, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux);
+            k_moments<R, Dev, 3, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux, xg);
         else if (le)
-            k_moments<R, Dev, 2, true><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux);
+            k_moments<R, Dev, 2, true><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux, xg);
         else
-            k_moments<R, Dev, 2, false><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux);
+            k_moments<R, Dev, 2, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg);
     }
     void dipole(const DevBufs &b, hipStream_t s) override
     {
-        if (C == 3) k_dipole<R, Dev><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db);
+        if (C == 3) k_dipole<R, Dev><<<tiles(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db, xg);
     }
     template <int BD>
     void collide_bd(const DevBufs &b, hipStream_t s)
@@ -182,15 +184,15 @@ class LauncherT : public Launcher {
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (C == 3)
-            k_collide<R, Dev, 3, false, BD><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      (const R *)b.db, fo);
+            k_collide<R, Dev, 3, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                      (const R *)b.db, fo, xg);
         else if (le)
-            k_collide<R, Dev, 2, true, BD><<<blocks(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc,
+            k_collide<R, Dev, 2, true, BD><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc,
                                                                      (const R *)b.aux, (const R *)b.face_aux,
-                                                                     nullptr, fo);
+                                                                     nullptr, fo, xg);
         else
-            k_collide<R, Dev, 2, false, BD><<<blocks(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      nullptr, fo);
+            k_collide<R, Dev, 2, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
+                                                                      nullptr, fo, xg);
     }
     void collide(const DevBufs &b, hipStream_t s) override
     {
@@ -232,14 +234,14 @@ class LauncherT : public Launcher {
     template <int TX, int TY, bool ST>
     void fused_go(const DevBufs &b, int lz, hipStream_t s)
     {
-        dim3 grid((P.nx + TX - 1) / TX, (P.ny + TY - 1) / TY, (P.nz + lz - 1) / lz);
+        unsigned grid = xcd_round(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY) * ((P.nz + lz - 1) / lz));
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (le)
             k_fused2<R, Dev, true, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
-                                                                         (const R *)b.face_aux, fo, lz);
+                                                                         (const R *)b.face_aux, fo, lz, xg);
         else
-            k_fused2<R, Dev, false, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz);
+            k_fused2<R, Dev, false, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz, xg);
     }
     bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, bool st, hipStream_t s, bool launch)
     {
@@ -311,27 +313,27 @@ class BatchLauncherT : public BatchLauncher {
         const uint2 *it = (const uint2 *)items;
         if (ph == 0) {
             if (ncomp == 3)
-                k_batch_moments<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)db, (R *)aux);
+                k_batch_moments<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)db, (R *)aux, xg);
             else
-                k_batch_moments<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, nullptr, (R *)aux);
+                k_batch_moments<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, nullptr, (R *)aux, xg);
         } else if (ph == 1) {
-            if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db);
+            if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db, xg);
         } else {
             // every lattice's collide runs in the launch of its own equilibrium form
             for (int bd : {0, 2}) {
                 if (!(bd_mask & (1 << bd))) continue;
                 if (ncomp == 3 && bd == 0)
                     k_batch_collide<R, Dev, 3, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         (const R *)db, (R *)fout);
+                                                                         (const R *)db, (R *)fout, xg);
                 else if (ncomp == 3)
                     k_batch_collide<R, Dev, 3, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         (const R *)db, (R *)fout);
+                                                                         (const R *)db, (R *)fout, xg);
                 else if (bd == 0)
                     k_batch_collide<R, Dev, 2, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         nullptr, (R *)fout);
+                                                                         nullptr, (R *)fout, xg);
                 else
                     k_batch_collide<R, Dev, 2, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         nullptr, (R *)fout);
+                                                                         nullptr, (R *)fout, xg);
             }
         }
     }

@@ -24,6 +24,9 @@ struct DevBufs {
 class Launcher {
   public:
     virtual ~Launcher() {}
+    // chunk of consecutive tiles per XCD in the stepping kernels (xcd_tile);
+    // 1 is the plain round-robin order
+    uint32_t xg = 1;
     virtual void moments(const DevBufs &b, hipStream_t s) = 0;
     virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
     virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
@@ -53,6 +56,7 @@ constexpr unsigned BLOCK = 256;
 class BatchLauncher {
   public:
     virtual ~BatchLauncher() {}
+    uint32_t xg = 0;  // chunk of consecutive work items per XCD (xcd_tile)
     virtual size_t lat_bytes() const = 0;   // sizeof(BatchLat<R>)
     virtual void set_bdist_mask(int mask) = 0;
     // host image of the lattice table entry k

@@ -96,7 +96,7 @@ def lib():
         L.lb3d_gpu_get_density.argtypes = [vp, dp]
         L.lb3d_gpu_synchronize.argtypes = [vp]
         L.lb3d_gpu_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
-        L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip]
+        L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip, ip]
         L.lb3d_gpu_profile_steps.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_char_p), dp,
                                              ctypes.POINTER(ip), dp]
         L.lb3d_gpu_batch_create.argtypes = [ip, ctypes.POINTER(Params), ctypes.POINTER(vp)]
@@ -108,7 +108,7 @@ def lib():
         L.lb3d_gpu_batch_synchronize.argtypes = [vp]
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
         L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
-        L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip]
+        L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip, ip]
         L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
         _lib = L
@@ -209,8 +209,9 @@ class Lattice:
         check(lib().lb3d_gpu_autotune(self._h, int(trial_steps), buf, len(buf)), "autotune")
         return buf.value.decode()
 
-    def set_plan(self, fused, tx=64, ty=4, lz=64):
-        check(lib().lb3d_gpu_set_plan(self._h, int(fused), int(tx), int(ty), int(lz)), "set_plan")
+    def set_plan(self, fused, tx=64, ty=4, lz=64, xcd_group=1):
+        check(lib().lb3d_gpu_set_plan(self._h, int(fused), int(tx), int(ty), int(lz), int(xcd_group)),
+              "set_plan")
 
     def synchronize(self):
         check(lib().lb3d_gpu_synchronize(self._h), "synchronize")
@@ -347,8 +348,9 @@ class Batch:
         check(lib().lb3d_gpu_batch_diagnostics(self._h, k, m, p, c), "batch_diagnostics")
         return np.array(m[:]), np.array(p[:]), np.array(c[:])
 
-    def set_plan(self, sequential, nwg=2048):
-        check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg)), "batch_set_plan")
+    def set_plan(self, sequential, nwg=2048, xcd_group=0):
+        check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg), int(xcd_group)),
+              "batch_set_plan")
 
     def autotune(self, trial_steps=3):
         buf = ctypes.create_string_buffer(8192)

@@ -51,11 +51,12 @@ def test_batch_persistent_equals_sequential_and_single(precision):
             b.init_noise(k, 7 + k, amp=(0.01, 0.01, 0.01))
         return b
 
-    b1, b2, b3 = make(), make(), make()
+    b1, b2, b3, b4 = make(), make(), make(), make()
     b1.set_plan(0, 7)       # persistent, a small grid that strides many times
     b2.set_plan(1)          # lattice by lattice
     b3.autotune(2)          # whatever the tuner picks; the state must be unchanged
-    for b in (b1, b2, b3):
+    b4.set_plan(0, 24, 3)   # persistent, chunks of 3 work items per XCD
+    for b in (b1, b2, b3, b4):
         b.time_loop(9)
     for k, d in enumerate(DIMS):
         g = Lattice(*d, ncomp=ncomp, precision=precision, **per[k])
@@ -63,5 +64,5 @@ def test_batch_persistent_equals_sequential_and_single(precision):
         g.bc_init()
         g.time_loop(9)
         want = g.populations()
-        for b in (b1, b2, b3):
+        for b in (b1, b2, b3, b4):
             np.testing.assert_array_equal(b.populations(k), want)

@@ -48,6 +48,27 @@ def test_fused_equals_two_pass_bitwise(precision, dims, le):
                 f"{dims} le={le} mode {mode} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
 
 
+@pytest.mark.parametrize("ncomp,precision", [(2, LB3D_FP32_DEV), (3, LB3D_FP64)])
+def test_xcd_grouping_does_not_change_results(ncomp, precision):
+    """The work-group -> tile placement (chunks of xcd_group tiles per XCD)
+    covers every tile once for any chunk size, including a last chunk that is
+    narrower than the others (48x40x36: 270 site tiles, 34 per XCD)."""
+    dims = (48, 40, 36)
+    kw = dict(ncomp=ncomp, precision=precision, g_br=0.09, amp=(0.02, 0.02, 0.01))
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0, xcd_group=1)
+    ref.time_loop(5)
+    want = ref.populations()
+    plans = [(0, 64, 4, 1, xg) for xg in (3, 4, 0, 1000)]
+    if ncomp == 2:
+        plans += [(1, 32, 8, 5, xg) for xg in (1, 3, 0)] + [(1, 128, 4, 36, 0)]
+    for mode, tx, ty, lz, xg in plans:
+        _, g = pair(*dims, **kw)
+        g.set_plan(mode, tx, ty, lz, xcd_group=xg)
+        g.time_loop(5)
+        assert np.array_equal(g.populations(), want), f"plan {mode} {tx}x{ty} lz={lz} xcd_group={xg}"
+
+
 def test_one_pass_plan_rejected_where_it_does_not_exist():
     for kw in (dict(ncomp=3), dict(ncomp=2, bdist=0)):
         g = Lattice(8, 8, 8, **kw)
