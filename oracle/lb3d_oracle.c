@@ -26,6 +26,9 @@
  *                              feq(rho, (0,0,-+2U)), z-linear field shift
  *   body force               : bodyForce golden dumps
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "lb3d_oracle.h"
 
 #include <math.h>
@@ -104,6 +107,26 @@ void or_destroy(or_state *o)
     free(o->buf);
     free(o->force);
     free(o);
+}
+
+/* Worker threads of the site loops (OpenMP; every site is computed exactly
+ * as with one thread, so results do not depend on the count). */
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 double *or_sites(or_state *o) { return o->N; }
@@ -202,6 +225,7 @@ static void update_moments_site(or_state *o, double *S)
 /* lb3d_update_density_loop / lb3d_update_velocity_loop, code/lb3d_analysis.F90:107-153 */
 static void update_moments_loop(or_state *o)
 {
+    #pragma omp parallel for schedule(static)
     for (int z = 0; z < o->lz; z++)
         for (int y = 0; y < o->ly; y++)
             for (int x = 0; x < o->lx; x++) update_moments_site(o, site(o, x, y, z));
@@ -443,6 +467,7 @@ void or_advection(or_state *o)
 {
     int nx = o->p.nx, ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
     memcpy(o->buf, o->N, sizeof(double) * (size_t)o->lx * o->ly * o->lz * o->sd);
+    #pragma omp parallel for schedule(static)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {
@@ -614,6 +639,7 @@ void or_collision(or_state *o)
     int nx = o->p.nx, ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
     if (C == 3) {
         /* :114-141 */
+        #pragma omp parallel for schedule(static)
         for (int z = 1; z <= nz; z++)
             for (int y = 1; y <= ny; y++)
                 for (int x = 1; x <= nx; x++) dipole_collision(o, x, y, z);
@@ -621,6 +647,7 @@ void or_collision(or_state *o)
     }
     /* force loop, :151-177 */
     size_t ns = (size_t)nx * ny * nz;
+    #pragma omp parallel for schedule(static)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {
@@ -631,6 +658,7 @@ void or_collision(or_state *o)
                     for (int k = 0; k < 3; k++) o->force[(c * ns + i) * 3 + k] = f[c][k];
             }
     /* collide loop, :188-289 */
+    #pragma omp parallel for schedule(static)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {
@@ -874,6 +902,7 @@ void or_init_noise(or_state *o, uint64_t seed, const double rho_mean[3], const d
 {
     int nx = o->p.nx, ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
     const double w_move = 1.0 / 36.0, w_rest = 1.0 / 3.0;
+    #pragma omp parallel for schedule(static)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {

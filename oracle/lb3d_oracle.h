@@ -44,6 +44,11 @@ typedef struct or_params {
 
 typedef struct or_state or_state;
 
+/* Worker threads of the site loops (OpenMP build); results are identical for
+ * every thread count because each site is computed on its own. */
+void or_set_threads(int n);
+int  or_threads(void);
+
 or_state *or_create(const or_params *p);
 void      or_destroy(or_state *o);
 double   *or_sites(or_state *o);             /* whole_N, (nx+2)(ny+2)(nz+2)*site_doubles */

@@ -71,8 +71,19 @@ def lib():
         L.or_init_noise.argtypes = [vp, ctypes.c_uint64, dp, dp, ip]
         L.or_hash_uniform.restype = ctypes.c_double
         L.or_hash_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+        L.or_set_threads.argtypes = [ip]
+        L.or_threads.restype = ip
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Worker threads of the oracle's site loops (results do not depend on it)."""
+    lib().or_set_threads(int(n))
+
+
+def threads():
+    return int(lib().or_threads())
 
 
 DEFAULTS = dict(
