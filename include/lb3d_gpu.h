@@ -165,6 +165,20 @@ int lb3d_gpu_get_populations(lb3d_ctx *ctx, double *out);
  * amass).  Synchronises. */
 int lb3d_gpu_get_density(lb3d_ctx *ctx, double *out);
 
+/* Momenta P_c = sum_i g_i n_c,i c_i of the interior in FP64 (the u field of
+ * lbe_site, lb3d_update_velocity, code/lb3d_analysis.F90:78-102),
+ * out[((c*3 + a)*nz + z)*ny*nx + y*nx + x].  Synchronises. */
+int lb3d_gpu_get_momentum(lb3d_ctx *ctx, double *out);
+
+/* Lees-Edwards x-halo planes that the pre-advection boundary pass builds from
+ * the current populations with the current offset (lelo, lefrac of
+ * lb3d_gpu_le_state): the interpolated, velocity-shifted and clamped planes of
+ * le_halo_exchange (code/lb3d_bc_leesedwards.F90:659-1004), in FP64,
+ * out[((f*2 + c)*19 + i)*nz*ny + z*ny + y], f = 0 the x = 0 plane, f = 1 the
+ * x = nx+1 plane (y, z 0-based interior).  Binary Lees-Edwards lattices only;
+ * needs a completed timestep.  Synchronises. */
+int lb3d_gpu_le_halo(lb3d_ctx *ctx, double *out);
+
 /* Wait for all work queued on the context's stream. */
 int lb3d_gpu_synchronize(lb3d_ctx *ctx);
 

@@ -766,6 +766,74 @@ int lb3d_gpu_get_density(lb3d_ctx *c, double *out)
     return LB3D_OK;
 }
 
+int lb3d_gpu_get_momentum(lb3d_ctx *c, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    const int C = c->p.ncomp;
+    const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    std::vector<double> plane(ns);
+    std::vector<float> planef(is_dev(c) ? ns : 0);
+    const size_t rs = c->L->real_bytes();
+    for (int k = 0; k < C; k++) {
+        double *P = out + (size_t)k * 3 * ns;
+        for (size_t s = 0; s < 3 * ns; s++) P[s] = 0.0;
+        for (int i = 0; i < 19; i++) {
+            const char *src = (const char *)c->b.f[c->b.cur] + ((size_t)k * 19 + i) * ns * rs;
+            if (is_dev(c)) {
+                HIP_TRY(hipMemcpyAsync(planef.data(), src, ns * 4, hipMemcpyDeviceToHost, c->stream));
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                double base = w_of(i) * rho0_of(c, k);
+                for (size_t s = 0; s < ns; s++) plane[s] = base + (double)planef[s];
+            } else {
+                HIP_TRY(hipMemcpyAsync(plane.data(), src, ns * 8, hipMemcpyDeviceToHost, c->stream));
+                HIP_TRY(hipStreamSynchronize(c->stream));
+            }
+            // P += g_i n_i c_i, i ascending (lb3d_update_velocity)
+            const double g = (double)host::GD[i];
+            const int cv[3] = {host::CX[i], host::CY[i], host::CZ[i]};
+            for (int a = 0; a < 3; a++) {
+                if (cv[a] == 0) continue;
+                double *Pa = P + (size_t)a * ns;
+                for (size_t s = 0; s < ns; s++) Pa[s] = Pa[s] + (plane[s] * g) * (double)cv[a];
+            }
+        }
+    }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_le_halo(lb3d_ctx *c, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    if (!c->le) return fail(LB3D_EINVAL, "not a Lees-Edwards lattice");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "le_halo needs a completed timestep");
+    int rc = sync_le_counter(c);
+    if (rc) return rc;
+    // the stage-6 x-halo planes of the current populations and offset
+    c->L->le_faces(c->b, false, c->stream);
+    HIP_TRY(hipGetLastError());
+    const size_t plane = (size_t)c->p.ny * c->p.nz;
+    const size_t n = 2 * 2 * 19 * plane;
+    const size_t rs = c->L->real_bytes();
+    if (is_dev(c)) {
+        std::vector<float> h(n);
+        HIP_TRY(hipMemcpyAsync(h.data(), c->b.face_pc, n * rs, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (size_t f = 0; f < 2; f++)
+            for (int k = 0; k < 2; k++)
+                for (int i = 0; i < 19; i++) {
+                    size_t o = ((f * 2 + k) * 19 + i) * plane;
+                    double base = w_of(i) * rho0_of(c, k);
+                    for (size_t t = 0; t < plane; t++) out[o + t] = base + (double)h[o + t];
+                }
+    } else {
+        HIP_TRY(hipMemcpyAsync(out, c->b.face_pc, n * rs, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return LB3D_OK;
+}
+
 int lb3d_gpu_get_nt(lb3d_ctx *c, int32_t *nt)
 {
     if (!c || !nt) return fail(LB3D_EINVAL, "NULL argument");

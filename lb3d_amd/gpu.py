@@ -34,7 +34,8 @@ EXPORTS = [
     "lb3d_gpu_bc_init", "lb3d_gpu_force_apply", "lb3d_gpu_bc_apply", "lb3d_gpu_advection",
     "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
     "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
-    "lb3d_gpu_get_density", "lb3d_gpu_synchronize", "lb3d_gpu_autotune", "lb3d_gpu_set_plan",
+    "lb3d_gpu_get_density", "lb3d_gpu_get_momentum", "lb3d_gpu_le_halo",
+    "lb3d_gpu_synchronize", "lb3d_gpu_autotune", "lb3d_gpu_set_plan",
     "lb3d_gpu_batch_create", "lb3d_gpu_batch_destroy", "lb3d_gpu_batch_init_noise",
     "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
@@ -94,6 +95,8 @@ def lib():
         L.lb3d_gpu_le_state.argtypes = [vp, dp, ctypes.POINTER(ip), dp, dp]
         L.lb3d_gpu_get_populations.argtypes = [vp, dp]
         L.lb3d_gpu_get_density.argtypes = [vp, dp]
+        L.lb3d_gpu_get_momentum.argtypes = [vp, dp]
+        L.lb3d_gpu_le_halo.argtypes = [vp, dp]
         L.lb3d_gpu_synchronize.argtypes = [vp]
         L.lb3d_gpu_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip, ip]
@@ -201,6 +204,60 @@ class Lattice:
         check(lib().lb3d_gpu_get_density(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               "get_density")
         return out
+
+    def momentum(self):
+        """Interior momenta P_c = sum_i g_i n_c,i c_i in FP64, shape (ncomp, 3, nz, ny, nx)."""
+        out = np.empty((self.ncomp, 3, self.nz, self.ny, self.nx))
+        check(lib().lb3d_gpu_get_momentum(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "get_momentum")
+        return out
+
+    def le_halo(self):
+        """Lees-Edwards x-halo planes that the pre-advection pass builds from the
+        current populations and offset, FP64, shape (2, 2, 19, nz, ny): plane
+        x = 0 then x = nx+1, component, direction, z, y."""
+        out = np.empty((2, 2, 19, self.nz, self.ny))
+        check(lib().lb3d_gpu_le_halo(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "le_halo")
+        return out
+
+    def shear_stress(self, minx=1, maxx=None):
+        """Momentum transfer through the planes x = minx and x = maxx (1-based),
+        the reference's local_fluid_momentum_transfer and dump_stress
+        (code/lb3d_io_stress.F90:38-173, one rank): vectors 9, 10 enter plane
+        minx from x = minx-1 and their bounces 13, 14 leave it; 13, 14 enter
+        plane maxx from maxx+1.  The x = 0 / nx+1 planes are the pre-advection
+        halo (le_halo for Lees-Edwards, periodic images otherwise).  Returns
+        (out0, out1, mean) with out halved as dump_stress does and
+        mean = (out1 - out0)/(ny*nz)."""
+        maxx = self.nx if maxx is None else maxx
+        if not (1 <= minx <= self.nx and 1 <= maxx <= self.nx):
+            raise ValueError("stress planes must lie in 1..nx")
+        pops = self.populations()
+        if self.params.boundary == LB3D_BC_LEES_EDWARDS:
+            halo = self.le_halo()
+        else:
+            halo = np.stack([pops[..., -1], pops[..., 0]])
+
+        def plane(x):  # populations of plane x (0..nx+1), shape (ncomp, 19, nz, ny)
+            if x == 0:
+                return halo[0]
+            if x == self.nx + 1:
+                return halo[1]
+            return pops[..., x - 1]
+
+        cz = (0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0)
+        bounce = (1, 0, 3, 2, 5, 4, 11, 10, 13, 12, 7, 6, 9, 8, 17, 16, 15, 14, 18)
+        out = []
+        for m, src, vecs in ((minx, minx - 1, (8, 9)), (maxx, maxx + 1, (12, 13))):
+            A, B = plane(src), plane(m)
+            tot = 0.0
+            for v in vecs:  # g = 1 on these diagonal vectors
+                r = bounce[v]
+                # A at z - cz_v (periodic in z): roll by +cz_v along z
+                tot += float((np.roll(A[:, v], cz[v], axis=1) * cz[v]).sum() - (B[:, r] * cz[r]).sum())
+            out.append(0.5 * tot)
+        return out[0], out[1], (out[1] - out[0]) / (self.ny * self.nz)
 
     def autotune(self, trial_steps=3):
         """Time every execution plan on this lattice and keep the fastest;

@@ -457,6 +457,48 @@ void or_bc_apply(or_state *o, int stage)
     }
 }
 
+/* The halo as the pre-advection pass builds it from the current populations,
+ * without advancing the Lees-Edwards offset (le_halo_exchange with the
+ * current lelo/lefrac; periodic copies otherwise). */
+void or_halo_refresh(or_state *o)
+{
+    if (o->p.lees_edwards)
+        le_halo_exchange(o);
+    else
+        halo_exchange_local(o);
+}
+
+/* Momentum transfer through the planes x = minx and x = maxx (global
+ * 1-based), local_fluid_momentum_transfer and dump_stress,
+ * code/lb3d_io_stress.F90:38-173 on one rank: vectors 9, 10 (pvecs) carry
+ * +x and z momentum into plane minx from x = minx-1, their bounces 13, 14
+ * leave it; mvecs 13, 14 enter plane maxx from maxx+1.  Summed over
+ * components, halved; the mean shear stress is (out[1] - out[0])/(ny*nz).
+ * Reads the halo as it stands (or_halo_refresh builds the pre-advection one). */
+void or_stress(const or_state *o, int minx, int maxx, double out[2])
+{
+    static const int pvecs[2] = {8, 9}, mvecs[2] = {12, 13}; /* 0-based 9,10 and 13,14 */
+    static const int bounce[19] = {1, 0, 3, 2, 5, 4, 11, 10, 13, 12, 7, 6, 9, 8, 17, 16, 15, 14, 18};
+    int ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
+    double lf[2] = {0.0, 0.0};
+    for (int side = 0; side < 2; side++) {
+        int m = side == 0 ? minx : maxx, src = side == 0 ? m - 1 : m + 1;
+        const int *vecs = side == 0 ? pvecs : mvecs;
+        for (int y = 1; y <= ny; y++)
+            for (int z = 1; z <= nz; z++)
+                for (int t = 0; t < 2; t++) {
+                    int sv = vecs[t], r = bounce[sv];
+                    const double *A = csite(o, src, y, z - cz[sv]);
+                    const double *B = csite(o, m, y, z);
+                    for (int c = 0; c < C; c++)
+                        lf[side] = lf[side] + A[OFF_N(c) + sv] * (double)cz[sv] * (double)gdeg[sv] -
+                                   B[OFF_N(c) + r] * (double)cz[r] * (double)gdeg[r];
+                }
+    }
+    out[0] = lf[0] * 0.5;
+    out[1] = lf[1] * 0.5;
+}
+
 /* ---------------------------------------------------------------------------
  * Advection, code/lb3d_advection.F90:54-151.  Pull streaming of the 18 moving
  * populations from the pre-advection lattice (the rolling 3-plane buffer Nbuf

@@ -66,6 +66,12 @@ void or_collision(or_state *o);              /* stage 9 */
 void or_time_step(or_state *o);
 void or_run(or_state *o, int nsteps);
 
+/* the pre-advection halo of the current state, offset not advanced */
+void or_halo_refresh(or_state *o);
+/* momentum transfer through x = minx and x = maxx (code/lb3d_io_stress.F90:38-173),
+ * halved as dump_stress does; mean shear stress = (out[1] - out[0])/(ny*nz) */
+void or_stress(const or_state *o, int minx, int maxx, double out[2]);
+
 /* LE scalars after the last le_neighbours call */
 void or_le_state(const or_state *o, double *shear_sum, int *lelo, double *lefrac,
                  double *shear_u);

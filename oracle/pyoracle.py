@@ -72,6 +72,8 @@ def lib():
         L.or_hash_uniform.restype = ctypes.c_double
         L.or_hash_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
         L.or_set_threads.argtypes = [ip]
+        L.or_halo_refresh.argtypes = [vp]
+        L.or_stress.argtypes = [vp, ip, ip, dp]
         L.or_threads.restype = ip
         _lib = L
     return _lib
@@ -176,6 +178,18 @@ class Oracle:
         lib().or_le_state(self._h, ctypes.byref(s), ctypes.byref(lo), ctypes.byref(f),
                           ctypes.byref(u))
         return s.value, lo.value, f.value, u.value
+
+    def halo_refresh(self):
+        """Rebuild the halo as the pre-advection pass would, offset not advanced."""
+        lib().or_halo_refresh(self._h)
+
+    def stress(self, minx=1, maxx=None):
+        """Halved momentum transfer through x = minx and x = maxx
+        (code/lb3d_io_stress.F90:38-173) with the halo as it stands; returns
+        (out0, out1, mean shear stress)."""
+        out = (ctypes.c_double * 2)()
+        lib().or_stress(self._h, int(minx), int(self.nx if maxx is None else maxx), out)
+        return out[0], out[1], (out[1] - out[0]) / (self.ny * self.nz)
 
     def sanity(self):
         m = (ctypes.c_double * 3)()

@@ -197,3 +197,42 @@ def test_hash_uniform_range_and_determinism():
     assert min(v) >= 0.0 and max(v) < 1.0
     assert hash_uniform(1, 0, 5) == hash_uniform(1, 0, 5)
     assert hash_uniform(1, 0, 5) != hash_uniform(2, 0, 5)
+
+
+def test_stress_of_an_equilibrium_field():
+    """Momentum transfer through x-planes (code/lb3d_io_stress.F90:88-173) for
+    equilibrium populations rho_c(x)*feq(u) with uniform u = (ux, 0, uz):
+    plane minx gets rho(minx-1)*(b9-b10) + rho(minx)*(b14-b13) per (y, z) and
+    component (1-based vectors), plane maxx rho(maxx+1)*(b13-b14) +
+    rho(maxx)*(b10-b9); halved.  For uniform rho both reduce to the textbook
+    flux Pi_xz = rho*ux*uz."""
+    nx, ny, nz = 6, 4, 5
+    ux, uz = 0.03, -0.02
+    b = boltz(2, (ux, 0.0, uz))
+    np.testing.assert_allclose(b[8] - b[9] + b[13] - b[12], ux * uz, rtol=1e-13)
+    rho_x = {0: 0.61 + 0.013 * np.arange(nx), 1: 0.52 + 0.007 * np.arange(nx) ** 2}
+    o = Oracle(nx, ny, nz, ncomp=2)
+    I = o.interior()
+    for c in (0, 1):
+        I[..., 24 * c:24 * c + 19] = rho_x[c][None, None, :, None] * b[None, None, None, :]
+    o.halo_refresh()
+
+    def rho(c, x):  # 1-based plane, periodic images at 0 and nx+1
+        return rho_x[c][(x - 1) % nx]
+
+    for minx, maxx in ((1, nx), (2, 5), (3, 3)):
+        s0, s1, mean = o.stress(minx, maxx)
+        e0 = sum(rho(c, minx - 1) * (b[8] - b[9]) + rho(c, minx) * (b[13] - b[12]) for c in (0, 1))
+        e1 = sum(rho(c, maxx + 1) * (b[12] - b[13]) + rho(c, maxx) * (b[9] - b[8]) for c in (0, 1))
+        np.testing.assert_allclose(s0, 0.5 * ny * nz * e0, rtol=1e-12)
+        np.testing.assert_allclose(s1, 0.5 * ny * nz * e1, rtol=1e-12)
+        np.testing.assert_allclose(mean, (s1 - s0) / (ny * nz), rtol=1e-14)
+    # uniform density: the flux is rho*ux*uz through either plane (with opposite signs)
+    o2 = Oracle(nx, ny, nz, ncomp=2)
+    I2 = o2.interior()
+    for c, r in ((0, 0.7), (1, 0.4)):
+        I2[..., 24 * c:24 * c + 19] = r * b
+    o2.halo_refresh()
+    s0, s1, _ = o2.stress()
+    np.testing.assert_allclose(s0, 0.5 * ny * nz * 1.1 * ux * uz, rtol=1e-12)
+    np.testing.assert_allclose(s1, -0.5 * ny * nz * 1.1 * ux * uz, rtol=1e-12)
