@@ -152,6 +152,27 @@ int lb3d_gpu_get_nt(lb3d_ctx *ctx, int32_t *nt);
  * sites with a negative population, sites with |u_c| > 0.1}.  Synchronises. */
 int lb3d_gpu_diagnostics(lb3d_ctx *ctx, double mass[3], double momentum[3], int64_t counts[3]);
 
+/* The full sanity check of code/lb3d_sanity.F90:79-384 (one rank), in FP64
+ * on the current populations.  Per species c (0 r, 1 b, 2 s) and axis a:
+ * the velocity v = amass_c * sum_i g_i n_c,i c_i,a / rho_c of every site with
+ * |rho_c| > 1e-12 is summed (vel_sum; divide by the site count for the
+ * reference's average), and its largest magnitude is kept with the global
+ * 1-based (x, y, z) of that site (max_pos[c][a]; among equal maxima the site
+ * last in the reference's loop, x outermost and z innermost; 0 if none). */
+typedef struct lb3d_sanity {
+    double mass[3];             /* total density per component */
+    double momentum[3];         /* total mass flow, sum_c amass_c * P_c */
+    double vel_sum[3][3];
+    double max_vel[3][3];
+    int32_t max_pos[3][3][3];
+    int32_t pad0;
+    int64_t fluid_sites;
+    int64_t nan_sites;          /* sites with a NaN population (found_NaN if > 0) */
+    int64_t negative_sites;     /* sites with a negative population */
+    int64_t fast_sites;         /* sites where some species is faster than 0.1 */
+} lb3d_sanity;
+int lb3d_gpu_sanity(lb3d_ctx *ctx, lb3d_sanity *out);
+
 /* Lees-Edwards scalars after the last le_neighbours (code/lb3d_bc_leesedwards.F90:86-140). */
 int lb3d_gpu_le_state(lb3d_ctx *ctx, double *shear_sum, int32_t *lelo, double *lefrac,
                       double *shear_u);
@@ -210,6 +231,8 @@ int lb3d_gpu_batch_upload_sites(lb3d_batch *b, int32_t k, const double *whole_N,
 int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps);
 int lb3d_gpu_batch_get_nt(lb3d_batch *b, int32_t *nt);
 int lb3d_gpu_batch_synchronize(lb3d_batch *b);
+/* The full sanity check of lattice k (as lb3d_gpu_sanity). */
+int lb3d_gpu_batch_sanity(lb3d_batch *b, int32_t k, lb3d_sanity *out);
 /* Populations of lattice k in FP64 (layout as lb3d_gpu_get_populations). */
 int lb3d_gpu_batch_get_populations(lb3d_batch *b, int32_t k, double *out);
 /* Sanity reductions of lattice k (as lb3d_gpu_diagnostics). */

@@ -178,7 +178,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
     if (b->C == 3 && (e = hipMalloc(&b->db, dbo * b->rs)) != hipSuccess) return fail_destroy("hipMalloc db", e);
     if ((e = hipMalloc(&b->lat, lat.size())) != hipSuccess) return fail_destroy("hipMalloc table", e);
     if ((e = hipMalloc(&b->items, items.size() * sizeof(uint2))) != hipSuccess) return fail_destroy("hipMalloc items", e);
-    if ((e = hipMalloc((void **)&b->partials, ((maxns + BLOCK - 1) / BLOCK) * NDIAG * sizeof(double))) != hipSuccess)
+    if ((e = hipMalloc((void **)&b->partials, ((maxns + BLOCK - 1) / BLOCK) * NSAN * sizeof(double))) != hipSuccess)
         return fail_destroy("hipMalloc partials", e);
     if ((e = hipMemcpy(b->lat, lat.data(), lat.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->items, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
@@ -289,6 +289,21 @@ int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double 
         momentum[j] = acc[3 + j];
         counts[j] = (int64_t)llround(acc[6 + j]);
     }
+    return LB3D_OK;
+}
+
+int lb3d_gpu_batch_sanity(lb3d_batch *b, int32_t k, lb3d_sanity *out)
+{
+    clear_error();
+    if (!b || k < 0 || k >= b->nlat || !out) return set_error(LB3D_EINVAL, "bad argument");
+    b->L[k]->sanity(view(b, k), b->partials, b->stream);
+    BTRY(hipGetLastError());
+    unsigned nb = b->L[k]->diag_blocks();
+    std::vector<double> h((size_t)nb * NSAN);
+    BTRY(hipMemcpyAsync(h.data(), b->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    BTRY(hipStreamSynchronize(b->stream));
+    const lb3d_params &p = b->p[k];
+    reduce_sanity(h.data(), nb, p.ncomp, p.nx, p.ny, p.nz, out);
     return LB3D_OK;
 }
 

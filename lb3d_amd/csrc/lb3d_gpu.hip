@@ -95,6 +95,46 @@ static int check_params(const lb3d_params *p)
 }
 
 namespace lb3d {
+void reduce_sanity(const double *h, unsigned nb, int ncomp, int nx, int ny, int nz, lb3d_sanity *out)
+{
+    std::memset(out, 0, sizeof *out);
+    double acc[SAN_MAX] = {0};
+    double mx[9], key[9];
+    for (int k = 0; k < 9; k++) mx[k] = key[k] = -1.0;
+    for (unsigned b = 0; b < nb; b++) {
+        const double *p = h + (size_t)b * NSAN;
+        for (int k = 0; k < SAN_MAX; k++) acc[k] = acc[k] + p[k];
+        for (int k = 0; k < 9; k++) {
+            double m = p[SAN_MAX + k], q = p[SAN_KEY + k];
+            if (m > mx[k] || (m == mx[k] && q > key[k])) {
+                mx[k] = m;
+                key[k] = q;
+            }
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        out->mass[k] = k < ncomp ? acc[k] : 0.0;
+        out->momentum[k] = acc[3 + k];
+    }
+    out->nan_sites = (int64_t)llround(acc[6]);
+    out->negative_sites = (int64_t)llround(acc[7]);
+    out->fast_sites = (int64_t)llround(acc[8]);
+    out->fluid_sites = (int64_t)llround(acc[SAN_FLUID]);
+    for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) {
+            int k = 3 * c + a;
+            out->vel_sum[c][a] = c < ncomp ? acc[SAN_VSUM + k] : 0.0;
+            if (c < ncomp && key[k] >= 0.0) {
+                long long q = llround(key[k]);
+                out->max_vel[c][a] = mx[k];
+                out->max_pos[c][a][0] = (int32_t)(q / ((long long)ny * nz)) + 1;
+                out->max_pos[c][a][1] = (int32_t)((q / nz) % ny) + 1;
+                out->max_pos[c][a][2] = (int32_t)(q % nz) + 1;
+            }
+        }
+    (void)nx;
+}
+
 int set_error(int code, const char *msg) { return fail(code, msg); }
 void clear_error() { g_err.clear(); }
 double rho0_of(const lb3d_params &p, int k) { return p.rho0[k] > 0.0 ? p.rho0[k] : 0.7; }
@@ -205,7 +245,7 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
             return cleanup(LB3D_ENOMEM, std::string("hipMalloc LE planes: ") + hipGetErrorString(e));
     }
     if ((e = hipMalloc((void **)&c->b.le, sizeof(LEState))) != hipSuccess ||
-        (e = hipMalloc((void **)&c->d_partials, (size_t)c->L->diag_blocks() * NDIAG * sizeof(double))) !=
+        (e = hipMalloc((void **)&c->d_partials, (size_t)c->L->diag_blocks() * NSAN * sizeof(double))) !=
             hipSuccess)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     if (p->ncomp == 2 && c->L->fused_supported(32, 4, true)) {
@@ -859,6 +899,21 @@ int lb3d_gpu_diagnostics(lb3d_ctx *c, double mass[3], double momentum[3], int64_
         momentum[k] = acc[3 + k];
         counts[k] = (int64_t)llround(acc[6 + k]);
     }
+    return LB3D_OK;
+}
+
+
+int lb3d_gpu_sanity(lb3d_ctx *c, lb3d_sanity *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    unsigned nb = c->L->diag_blocks();
+    c->L->sanity(c->b, c->d_partials, c->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nb * NSAN);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->d_partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    reduce_sanity(h.data(), nb, c->p.ncomp, c->p.nx, c->p.ny, c->p.nz, out);
     return LB3D_OK;
 }
 

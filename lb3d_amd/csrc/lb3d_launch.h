@@ -151,6 +151,83 @@ __global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restr
         for (int k = 0; k < NDIAG; k++) partials[(size_t)blockIdx.x * NDIAG + k] = red[k][0];
 }
 
+// Full sanity check (code/lb3d_sanity.F90:131-224) on the current
+// populations.  Per block, in FP64: mass[3], mass flow[3], NaN / negative /
+// too-fast / fluid site counts, the sums of the species velocities
+// vel[c][a] = amass_c * sum_i g_i n_c,i c_i,a / rho_c (|rho_c| > 1e-12), and
+// per (c, a) the largest |vel[c][a]| with its site.  Equal maxima keep the
+// site that comes last in the reference's loop (x outermost, z innermost), as
+// its `>=` update does: the key is that loop position, larger wins.
+// Partials are reduced on the host.
+template <typename R, bool Dev, int C>
+__global__ __launch_bounds__(256) void k_sanity(Phys<R> P, const R *__restrict__ f,
+                                                double *__restrict__ partials)
+{
+    __shared__ double red[NSAN][256];
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[NSAN];
+    for (int k = 0; k < NSAN; k++) v[k] = k < SAN_MAX ? 0.0 : -1.0;
+    if (s < P.ns) {
+        int x, y, z;
+        site_xyz(s, P.nx, P.ny, x, y, z);
+        double key = ((double)x * (double)P.ny + (double)y) * (double)P.nz + (double)z;
+        bool isnan_ = false, isneg = false, fast = false;
+        v[SAN_FLUID] = 1.0;
+        for (int c = 0; c < C; c++) {
+            double rho = 0.0, m[3] = {0.0, 0.0, 0.0};
+            for (int i = 0; i < 19; i++) {
+                double n = (double)f[((uint32_t)c * 19u + (uint32_t)i) * P.ns + s];
+                if (Dev) n = (i < 18 ? (double)P.rho0[c] / 36.0 : (double)P.rho0[c] / 3.0) + n;
+                if (n != n) isnan_ = true;
+                if (n < 0.0) isneg = true;
+                double ng = n * (double)GD[i];
+                rho = rho + ng;
+                m[0] = m[0] + ng * (double)CX[i];
+                m[1] = m[1] + ng * (double)CY[i];
+                m[2] = m[2] + ng * (double)CZ[i];
+            }
+            v[c] = rho;
+            double am = (double)P.amass[c];
+            for (int a = 0; a < 3; a++) v[3 + a] = v[3 + a] + am * m[a];
+            if (fabs(rho) > 1.0e-12) {
+                double u[3];
+                for (int a = 0; a < 3; a++) {
+                    u[a] = m[a] * am / rho;
+                    v[SAN_VSUM + 3 * c + a] = u[a];
+                    double au = fabs(u[a]);
+                    if (au >= 0.0) {  // false for NaN, which the reference's >= skips too
+                        v[SAN_MAX + 3 * c + a] = au;
+                        v[SAN_KEY + 3 * c + a] = key;
+                    }
+                }
+                if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) > 0.1) fast = true;
+            }
+        }
+        v[6] = isnan_ ? 1.0 : 0.0;
+        v[7] = isneg ? 1.0 : 0.0;
+        v[8] = fast ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < NSAN; k++) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const unsigned o = threadIdx.x + w;
+            for (int k = 0; k < SAN_MAX; k++) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][o];
+            for (int k = 0; k < 9; k++) {
+                double m0 = red[SAN_MAX + k][threadIdx.x], m1 = red[SAN_MAX + k][o];
+                double k0 = red[SAN_KEY + k][threadIdx.x], k1 = red[SAN_KEY + k][o];
+                if (m1 > m0 || (m1 == m0 && k1 > k0)) {
+                    red[SAN_MAX + k][threadIdx.x] = m1;
+                    red[SAN_KEY + k][threadIdx.x] = k1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NSAN; k++) partials[(size_t)blockIdx.x * NSAN + k] = red[k][0];
+}
+
 template <typename R, bool Dev>
 class LauncherT : public Launcher {
   public:
@@ -274,6 +351,14 @@ class LauncherT : public Launcher {
         else
             k_init_noise<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, seed, m[0], m[1], m[2], a[0], a[1], a[2], r0[0],
                                                                 r0[1], r0[2], dip, f, nullptr);
+    }
+    void sanity(const DevBufs &b, double *partials, hipStream_t s) override
+    {
+        const R *f = (const R *)b.f[b.cur];
+        if (C == 3)
+            k_sanity<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+        else
+            k_sanity<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
     }
     void diagnostics(const DevBufs &b, double *partials, hipStream_t s) override
     {

@@ -40,6 +40,8 @@ class Launcher {
                             const double amp[3], int dipoles, hipStream_t s) = 0;
     // per-block FP64 partial sums for the sanity reductions (NDIAG per block)
     virtual void diagnostics(const DevBufs &b, double *partials, hipStream_t s) = 0;
+    // per-block partials of the full sanity check (NSAN per block)
+    virtual void sanity(const DevBufs &b, double *partials, hipStream_t s) = 0;
     virtual unsigned diag_blocks() const = 0;
     virtual size_t real_bytes() const = 0;
 };
@@ -49,6 +51,11 @@ constexpr int FUSED_SHAPES[][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {128, 4}
 constexpr int N_FUSED_SHAPES = sizeof(FUSED_SHAPES) / sizeof(FUSED_SHAPES[0]);
 
 constexpr int NDIAG = 9;  // mass[3], momentum[3], nan sites, negative sites, fast sites
+// full sanity partials: NDIAG values, fluid sites, velocity sums [3][3], then
+// per (species, axis) the largest |velocity| and its loop-order key
+constexpr int SAN_FLUID = 9, SAN_VSUM = 10, SAN_MAX = 19, SAN_KEY = 28, NSAN = 37;
+// fill *out from nb blocks of k_sanity partials (host)
+void reduce_sanity(const double *partials, unsigned nb, int ncomp, int nx, int ny, int nz, lb3d_sanity *out);
 constexpr unsigned BLOCK = 256;
 
 // Batched ensemble kernels of one precision.  The lattice table (BatchLat<R>

@@ -33,7 +33,19 @@ module lb3d_gpu_module
      real(c_double) :: rho0(3)
   end type lb3d_params
 
+  !> lb3d_sanity: C [species][axis][xyz] arrays appear here as (xyz, axis, species)
+  type, bind(C), public :: lb3d_sanity
+     real(c_double) :: mass(3)
+     real(c_double) :: momentum(3)
+     real(c_double) :: vel_sum(3, 3)
+     real(c_double) :: max_vel(3, 3)
+     integer(c_int32_t) :: max_pos(3, 3, 3)
+     integer(c_int32_t) :: pad0
+     integer(c_int64_t) :: fluid_sites, nan_sites, negative_sites, fast_sites
+  end type lb3d_sanity
+
   public :: lb3d_gpu_abi_version, lb3d_gpu_create, lb3d_gpu_destroy
+  public :: lb3d_gpu_sanity
   public :: lb3d_gpu_upload_sites, lb3d_gpu_download_sites, lb3d_gpu_init_noise
   public :: lb3d_gpu_bc_init, lb3d_gpu_force_apply, lb3d_gpu_bc_apply
   public :: lb3d_gpu_advection, lb3d_gpu_collision, lb3d_gpu_run_steps
@@ -147,6 +159,13 @@ module lb3d_gpu_module
        integer(c_int64_t), intent(out) :: counts(3)
        integer(c_int32_t) :: rc
      end function lb3d_gpu_diagnostics
+
+     function lb3d_gpu_sanity(ctx, rep) bind(C, name="lb3d_gpu_sanity") result(rc)
+       import :: c_int32_t, c_ptr, lb3d_sanity
+       type(c_ptr), value :: ctx
+       type(lb3d_sanity), intent(out) :: rep
+       integer(c_int32_t) :: rc
+     end function lb3d_gpu_sanity
 
      function lb3d_gpu_le_state(ctx, shear_sum, lelo, lefrac, shear_u) &
           bind(C, name="lb3d_gpu_le_state") result(rc)

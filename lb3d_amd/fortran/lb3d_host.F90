@@ -248,6 +248,9 @@ contains
   subroutine write_checkpoint_xdr()
     character(len=512) :: fname
     integer :: u, x, y, z, rec, k
+    integer(1) :: topo(32)
+    real(c_double) :: ss, lefrac, su
+    integer(c_int32_t) :: lelo
     integer, allocatable :: off(:)
     integer(1), allocatable :: bytes(:)
     real(rk), allocatable :: vals(:)
@@ -271,6 +274,18 @@ contains
     write(u) bytes
     close(u)
     call log_msg('Wrote checkpoint '//trim(fname))
+    ! checktopo: cdims(3) = 1, ccoords(3) = 0 (one rank) and shear_sum
+    ! (dump_lattice_MPrxdr, code/lb3d_io_checkpoint.F90:763-774)
+    call check(lb3d_gpu_le_state(ctx, ss, lelo, lefrac, su), 'le_state')
+    topo(1:24) = transfer((/1_4, 1_4, 1_4, 0_4, 0_4, 0_4/), topo(1:24))
+    call swap4(topo(1:24))
+    topo(25:32) = transfer(ss, topo(25:32))
+    call swap8(topo(25:32))
+    write(fname, '(A,"/",A,"/checktopo_",A,"_t",I8.8,"-",I10.10,".xdr")') trim(folder), &
+         trim(cpfolder), trim(gr_out_file), nt, 0
+    open(newunit=u, file=fname, access='stream', form='unformatted', status='replace', action='write')
+    write(u) topo
+    close(u)
   end subroutine write_checkpoint_xdr
 
   subroutine dump_scalar_xdr(name, field)
@@ -455,25 +470,67 @@ module lb3d_sanity_module
   use lb3d_host_io
   implicit none
 contains
+  !> lbe_sanity_check (code/lb3d_sanity.F90:79-384): the same report, from
+  !> the device reductions of lb3d_gpu_sanity.
   subroutine lb3d_sanity_check()
-    real(c_double) :: mass(3), mom(3)
-    integer(c_int64_t) :: counts(3)
+    type(lb3d_sanity) :: r
     character(len=256) :: msg
+    character(len=4), parameter :: spec_name(3) = (/' red', 'blue', 'surf'/)
+    character(len=17), parameter :: avg_name(3) = (/'      red (x,y,z)', '     blue (x,y,z)', &
+         '     surf (x,y,z)'/)
+    real(rk) :: tot, sumdens, vmax(3)
+    integer :: i, nspec
     if (n_sanity_check <= 0) return
     if (mod(nt, n_sanity_check) /= 0) return
-    call check(lb3d_gpu_diagnostics(ctx, mass, mom, counts), 'lb3d_sanity_check')
-    write(msg, '("sanity nt=",I0," mass=",3ES20.12," momentum=",3ES12.4)') nt, mass, mom
+    call check(lb3d_gpu_sanity(ctx, r), 'lb3d_sanity_check')
+    nspec = ncomp
+    tot = real(nx, rk) * real(ny, rk) * real(nz, rk)
+    call log_msg('---------( Start LBE sanity check )----------')
+    if (r%nan_sites > 0) call log_msg('Found NaN')
+    write(msg, "('Total densities (r,b,g)   : ',F16.4,' ',F16.4,' ',F16.4)") &
+         max(0.d-30, r%mass(1)), max(0.d-15, r%mass(2)), max(0.d-30, r%mass(3))
     call log_msg(msg)
-    if (counts(2) > 0) then
-       write(msg, '("WARNING: ",I0," sites with negative populations")') counts(2)
+    write(msg, "('Average densities (r,b,g) : ',F16.10,' ',F16.10,' ',F16.10)") &
+         max(0.d-30, r%mass(1)/tot), max(0.d-30, r%mass(2)/tot), max(0.d-30, r%mass(3)/tot)
+    call log_msg(msg)
+    sumdens = r%mass(1) + r%mass(2) + r%mass(3)
+    write(msg, "('Total/average density     : ',F16.4,' ',F16.10)") sumdens, max(0.d-30, sumdens/tot)
+    call log_msg(msg)
+    call log_msg('Maximum absolute velocity : ')
+    do i = 1, nspec
+       vmax = r%max_vel(:, i)
+       write(msg, "('     ',A4,' (x,y,z)         : ',3(F16.10,:,' '))") spec_name(i), max(0.0e-30_rk, vmax)
+       call log_msg(msg)
+       write(msg, "('      at position',9X,': ',3('(',2(I4,','),I4,')',:,X))") &
+            r%max_pos(:, 1, i), r%max_pos(:, 2, i), r%max_pos(:, 3, i)
+       call log_msg(msg)
+    end do
+    call log_msg('Average velocity          : ')
+    do i = 1, nspec
+       write(msg, "(A17,'         : ',F16.10,' ',F16.10,' ',F16.10)") avg_name(i), r%vel_sum(:, i) / tot
+       call log_msg(msg)
+    end do
+    do i = 1, nspec
+       if (any(r%max_vel(:, i) > 0.1_rk)) &
+            call log_msg('********  WARNING: MAXIMUM '//trim(spec_name(i))//' VELOCITY IS TOO HIGH!  ********')
+    end do
+    if (r%fast_sites /= 0) then
+       write(msg, "('********  WARNING: absolute species velocity higher than ',F16.12,' on ',F16.2," &
+            //"' lattice sites ( ',F16.12,' % of total fluid sites)  ********')") &
+            0.1_rk, real(r%fast_sites, rk), 100.0_rk * real(r%fast_sites, rk) / real(r%fluid_sites, rk)
        call log_msg(msg)
     end if
-    if (counts(3) > 0) then
-       write(msg, '("WARNING: ",I0," sites faster than 0.1")') counts(3)
+    if (r%negative_sites /= 0) then
+       write(msg, "('********  WARNING: negative distribution functions on ',F16.2," &
+            //"' lattice sites ( ',F16.12,' % of total fluid sites)  ********')") &
+            real(r%negative_sites, rk), 100.0_rk * real(r%negative_sites, rk) / real(r%fluid_sites, rk)
        call log_msg(msg)
     end if
-    if (counts(1) > 0) then
-       call log_msg('Found NaN')
+    write(msg, "('Total momentum (x,y,z)    : ',3(F16.8,:,' '))") r%momentum
+    call log_msg(msg)
+    call log_msg('----------( End LBE sanity check )-----------')
+    if (r%nan_sites > 0) then
+       ! lb3d_time_loop's insanity branch: dump, checkpoint and stop
        call lb3d_io_write_data()
        call lb3d_io_write_checkpoint()
        error stop 2

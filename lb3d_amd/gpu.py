@@ -34,13 +34,31 @@ EXPORTS = [
     "lb3d_gpu_bc_init", "lb3d_gpu_force_apply", "lb3d_gpu_bc_apply", "lb3d_gpu_advection",
     "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
     "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
-    "lb3d_gpu_get_density", "lb3d_gpu_get_momentum", "lb3d_gpu_le_halo",
+    "lb3d_gpu_get_density", "lb3d_gpu_get_momentum", "lb3d_gpu_le_halo", "lb3d_gpu_sanity",
+    "lb3d_gpu_batch_sanity",
     "lb3d_gpu_synchronize", "lb3d_gpu_autotune", "lb3d_gpu_set_plan",
     "lb3d_gpu_batch_create", "lb3d_gpu_batch_destroy", "lb3d_gpu_batch_init_noise",
     "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
     "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_autotune", "lb3d_gpu_batch_profile_steps",
 ]
+
+
+class Sanity(ctypes.Structure):
+    """lb3d_sanity of include/lb3d_gpu.h."""
+    _fields_ = [("mass", ctypes.c_double * 3), ("momentum", ctypes.c_double * 3),
+                ("vel_sum", (ctypes.c_double * 3) * 3), ("max_vel", (ctypes.c_double * 3) * 3),
+                ("max_pos", ((ctypes.c_int32 * 3) * 3) * 3), ("pad0", ctypes.c_int32),
+                ("fluid_sites", ctypes.c_int64), ("nan_sites", ctypes.c_int64),
+                ("negative_sites", ctypes.c_int64), ("fast_sites", ctypes.c_int64)]
+
+    def as_dict(self):
+        return dict(mass=np.array(self.mass[:]), momentum=np.array(self.momentum[:]),
+                    vel_sum=np.array([r[:] for r in self.vel_sum]),
+                    max_vel=np.array([r[:] for r in self.max_vel]),
+                    max_pos=np.array([[p[:] for p in r] for r in self.max_pos]),
+                    fluid_sites=self.fluid_sites, nan_sites=self.nan_sites,
+                    negative_sites=self.negative_sites, fast_sites=self.fast_sites)
 
 
 class LB3DError(RuntimeError):
@@ -97,6 +115,8 @@ def lib():
         L.lb3d_gpu_get_density.argtypes = [vp, dp]
         L.lb3d_gpu_get_momentum.argtypes = [vp, dp]
         L.lb3d_gpu_le_halo.argtypes = [vp, dp]
+        L.lb3d_gpu_sanity.argtypes = [vp, ctypes.POINTER(Sanity)]
+        L.lb3d_gpu_batch_sanity.argtypes = [vp, ip, ctypes.POINTER(Sanity)]
         L.lb3d_gpu_synchronize.argtypes = [vp]
         L.lb3d_gpu_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_set_plan.argtypes = [vp, ip, ip, ip, ip, ip]
@@ -204,6 +224,12 @@ class Lattice:
         check(lib().lb3d_gpu_get_density(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               "get_density")
         return out
+
+    def sanity(self):
+        """The full sanity check (code/lb3d_sanity.F90:79-384) as a dict."""
+        r = Sanity()
+        check(lib().lb3d_gpu_sanity(self._h, ctypes.byref(r)), "sanity")
+        return r.as_dict()
 
     def momentum(self):
         """Interior momenta P_c = sum_i g_i n_c,i c_i in FP64, shape (ncomp, 3, nz, ny, nx)."""
@@ -404,6 +430,12 @@ class Batch:
         c = (ctypes.c_int64 * 3)()
         check(lib().lb3d_gpu_batch_diagnostics(self._h, k, m, p, c), "batch_diagnostics")
         return np.array(m[:]), np.array(p[:]), np.array(c[:])
+
+    def sanity(self, k):
+        """The full sanity check of lattice k as a dict."""
+        r = Sanity()
+        check(lib().lb3d_gpu_batch_sanity(self._h, int(k), ctypes.byref(r)), "batch_sanity")
+        return r.as_dict()
 
     def set_plan(self, sequential, nwg=2048, xcd_group=0):
         check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg), int(xcd_group)),

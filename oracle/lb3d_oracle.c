@@ -820,6 +820,69 @@ void or_sanity(const or_state *o, double mass[3], double mom[3], int64_t counts[
     counts[2] = nfast;
 }
 
+/* The full sanity check, code/lb3d_sanity.F90:79-384 on one rank, in its loop
+ * order (x outermost, z innermost): per species the velocity of each site
+ * with |rho| > 1e-12, its running sum, and the largest |component| with the
+ * site of the last `>=` update; totals and site counts as or_sanity. */
+void or_sanity_full(const or_state *o, or_sanity_report *r)
+{
+    int C = o->p.ncomp;
+    memset(r, 0, sizeof *r);
+    double maxvel[3][3] = {{0}};
+    for (int i = 1; i <= o->p.nx; i++)
+        for (int j = 1; j <= o->p.ny; j++)
+            for (int k = 1; k <= o->p.nz; k++) {
+                const double *S = csite(o, i, j, k);
+                r->fluid_sites += 1;
+                int isnan_site = 0, isneg = 0, fast = 0;
+                for (int c = 0; c < C; c++)
+                    for (int s = 0; s < 19; s++) {
+                        if (isnan(S[OFF_N(c) + s])) isnan_site = 1;
+                        if (S[OFF_N(c) + s] < 0.0) isneg = 1;
+                    }
+                for (int c = 0; c < C; c++) {
+                    double rho = 0.0;
+                    for (int s = 0; s < 19; s++) rho = rho + S[OFF_N(c) + s] * (double)gdeg[s];
+                    r->mass[c] = r->mass[c] + rho;
+                    if (fabs(rho) > 1.0e-12) {
+                        double m[3] = {0, 0, 0}, v[3];
+                        for (int s = 0; s < 19; s++) {
+                            double ng = S[OFF_N(c) + s] * (double)gdeg[s];
+                            m[0] = m[0] + ng * (double)cx[s];
+                            m[1] = m[1] + ng * (double)cy[s];
+                            m[2] = m[2] + ng * (double)cz[s];
+                        }
+                        for (int a = 0; a < 3; a++) {
+                            v[a] = m[a] * o->p.amass[c] / rho;
+                            r->vel_sum[c][a] = r->vel_sum[c][a] + v[a];
+                            if (fabs(v[a]) >= maxvel[c][a]) {
+                                maxvel[c][a] = fabs(v[a]);
+                                r->max_pos[c][a][0] = i;
+                                r->max_pos[c][a][1] = j;
+                                r->max_pos[c][a][2] = k;
+                            }
+                        }
+                        if (sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > 0.1) fast = 1;
+                    }
+                }
+                for (int c = 0; c < C; c++) {
+                    double m[3] = {0, 0, 0};
+                    for (int s = 0; s < 19; s++) {
+                        double ng = S[OFF_N(c) + s] * (double)gdeg[s];
+                        m[0] = m[0] + ng * (double)cx[s];
+                        m[1] = m[1] + ng * (double)cy[s];
+                        m[2] = m[2] + ng * (double)cz[s];
+                    }
+                    for (int a = 0; a < 3; a++) r->momentum[a] = r->momentum[a] + o->p.amass[c] * m[a];
+                }
+                r->nan_sites += isnan_site;
+                r->negative_sites += isneg;
+                r->fast_sites += fast;
+            }
+    for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) r->max_vel[c][a] = maxvel[c][a];
+}
+
 /* ---------------------------------------------------------------------------
  * Dump observables, code/lb3d_io_dump_data.F90, interior, x fastest.
  */

@@ -72,6 +72,17 @@ void or_halo_refresh(or_state *o);
  * halved as dump_stress does; mean shear stress = (out[1] - out[0])/(ny*nz) */
 void or_stress(const or_state *o, int minx, int maxx, double out[2]);
 
+/* full sanity check (code/lb3d_sanity.F90:79-384); species c, axis a */
+typedef struct or_sanity_report {
+    double mass[3], momentum[3];
+    double vel_sum[3][3];      /* sum of site velocities amass*P/rho where |rho| > 1e-12 */
+    double max_vel[3][3];      /* largest |velocity component| */
+    int32_t max_pos[3][3][3];  /* 1-based (x,y,z) of the last >= update; 0 if none */
+    int32_t pad0;
+    int64_t fluid_sites, nan_sites, negative_sites, fast_sites;
+} or_sanity_report;
+void or_sanity_full(const or_state *o, or_sanity_report *r);
+
 /* LE scalars after the last le_neighbours call */
 void or_le_state(const or_state *o, double *shear_sum, int *lelo, double *lefrac,
                  double *shear_u);

@@ -42,6 +42,23 @@ def build():
 _lib = None
 
 
+class OrSanity(ctypes.Structure):
+    """or_sanity_report of lb3d_oracle.h."""
+    _fields_ = [("mass", ctypes.c_double * 3), ("momentum", ctypes.c_double * 3),
+                ("vel_sum", (ctypes.c_double * 3) * 3), ("max_vel", (ctypes.c_double * 3) * 3),
+                ("max_pos", ((ctypes.c_int32 * 3) * 3) * 3), ("pad0", ctypes.c_int32),
+                ("fluid_sites", ctypes.c_int64), ("nan_sites", ctypes.c_int64),
+                ("negative_sites", ctypes.c_int64), ("fast_sites", ctypes.c_int64)]
+
+    def as_dict(self):
+        return dict(mass=np.array(self.mass[:]), momentum=np.array(self.momentum[:]),
+                    vel_sum=np.array([r[:] for r in self.vel_sum]),
+                    max_vel=np.array([r[:] for r in self.max_vel]),
+                    max_pos=np.array([[p[:] for p in r] for r in self.max_pos]),
+                    fluid_sites=self.fluid_sites, nan_sites=self.nan_sites,
+                    negative_sites=self.negative_sites, fast_sites=self.fast_sites)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -74,6 +91,7 @@ def lib():
         L.or_set_threads.argtypes = [ip]
         L.or_halo_refresh.argtypes = [vp]
         L.or_stress.argtypes = [vp, ip, ip, dp]
+        L.or_sanity_full.argtypes = [vp, ctypes.POINTER(OrSanity)]
         L.or_threads.restype = ip
         _lib = L
     return _lib
@@ -190,6 +208,12 @@ class Oracle:
         out = (ctypes.c_double * 2)()
         lib().or_stress(self._h, int(minx), int(self.nx if maxx is None else maxx), out)
         return out[0], out[1], (out[1] - out[0]) / (self.ny * self.nz)
+
+    def sanity_full(self):
+        """The full sanity check (code/lb3d_sanity.F90:79-384) as a dict."""
+        r = OrSanity()
+        lib().or_sanity_full(self._h, ctypes.byref(r))
+        return r.as_dict()
 
     def sanity(self):
         m = (ctypes.c_double * 3)()

@@ -49,3 +49,8 @@ def test_too_large_lattice_rejected():
 def test_params_struct_layout():
     # the Fortran bind(C) type and the C struct must agree on this size
     assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3
+
+
+def test_sanity_struct_layout():
+    # lb3d_sanity: 24 doubles, 27 + 1 int32, 4 int64 (the Fortran bind(C) type mirrors it)
+    assert ctypes.sizeof(gpu.Sanity) == 24 * 8 + 28 * 4 + 4 * 8

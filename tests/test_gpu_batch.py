@@ -66,3 +66,7 @@ def test_batch_persistent_equals_sequential_and_single(precision):
         want = g.populations()
         for b in (b1, b2, b3, b4):
             np.testing.assert_array_equal(b.populations(k), want)
+        sb, sg = b1.sanity(k), g.sanity()
+        np.testing.assert_array_equal(sb["max_pos"], sg["max_pos"])
+        np.testing.assert_array_equal(sb["max_vel"], sg["max_vel"])
+        assert sb["fluid_sites"] == np.prod(d)

@@ -68,3 +68,38 @@ def test_fortran_host_staged_loop_equals_graph_loop(tmp_path):
         assert r.returncode == 0, r.stdout + r.stderr
         outs[fast] = np.fromfile(d / "output" / f"od_{name}_t00000010-0000000000.xdr", dtype=">f4")
     np.testing.assert_array_equal(outs[".true."], outs[".false."])
+
+
+@pytest.mark.parametrize("name,ncomp,precision", [("gyroid", 3, "fp64"), ("leesEdwards", 2, "fp64"),
+                                                  ("leesEdwards", 2, "fp32")])
+def test_fortran_host_restart_equivalence(tmp_path, name, ncomp, precision):
+    """10 steps straight against 5 steps, the host's own XDR checkpoint (with
+    checktopo and shear_sum), a restore and 5 more steps: identical dumps."""
+    gold = np.load(GOLD)
+    base = open(os.path.join(INPUTS, f"{name}.in")).read()
+    extra = f"\n&gpu_input\nprecision = '{precision}'\n/\n"
+
+    def run(d, text):
+        d.mkdir(parents=True, exist_ok=True)
+        write_checkpoint(str(d / "checkpoint"), name, gold)
+        (d / f"{name}.in").write_text(text + extra)
+        r = subprocess.run([EXE, "-f", f"{name}.in", "-c", str(ncomp)], cwd=d, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        return r
+
+    straight = tmp_path / "straight"
+    run(straight, base)
+    first = tmp_path / "first"
+    run(first, base.replace("n_checkpoint = 10000", "n_checkpoint = 5"))
+    ck = first / "output" / f"checkpoint_{name}_t00000005-0000000000_p000000.xdr"
+    assert ck.exists() and (first / "output" / f"checktopo_{name}_t00000005-0000000000.xdr").exists()
+    second = tmp_path / "second"
+    text = base.replace("srccpfolder = './checkpoint'", f"srccpfolder = '{first / 'output'}'")
+    text = text.replace("restore_string = 't00000000-0000000000'", "restore_string = 't00000005-0000000000'")
+    r = run(second, text)
+    assert "t00000005" in r.stdout
+    for ob in CASES[name][1]:
+        fn = f"{ob}_{name}_t00000010-0000000000.xdr"
+        np.testing.assert_array_equal(np.fromfile(second / "output" / fn, dtype=">f4"),
+                                      np.fromfile(straight / "output" / fn, dtype=">f4"), err_msg=ob)

@@ -84,3 +84,30 @@ def test_fp32_le_halo_close_to_fp64():
     s64, s32 = g64.shear_stress(), g32.shear_stress()
     print(f"mean shear stress FP64 {s64[2]:.6e} FP32 {s32[2]:.6e}")
     assert abs(s32[2] - s64[2]) <= 1e-3 * abs(s64[2]) + 1e-8
+
+
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_full_sanity_matches_oracle(ncomp):
+    o, g = pair(11, 9, 7, ncomp=ncomp, amp=(0.02, 0.02, 0.01))
+    o.run(6)
+    g.time_loop(6)
+    want, got = o.sanity_full(), g.sanity()
+    for k in ("fluid_sites", "nan_sites", "negative_sites", "fast_sites"):
+        assert got[k] == want[k], k
+    np.testing.assert_allclose(got["mass"], want["mass"], rtol=1e-13)
+    np.testing.assert_allclose(got["momentum"], want["momentum"], rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(got["vel_sum"], want["vel_sum"], rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(got["max_vel"], want["max_vel"], rtol=1e-12)
+    np.testing.assert_array_equal(got["max_pos"], want["max_pos"])
+
+
+@pytest.mark.parametrize("precision", [LB3D_FP64, LB3D_FP32_DEV])
+def test_full_sanity_ties_keep_the_last_site(precision):
+    """A uniform state: every site has the same velocity, and the maximum is
+    reported at the last site of the reference's loop, (nx, ny, nz)."""
+    g = Lattice(12, 10, 6, ncomp=2, precision=precision)
+    g.init_noise(1, amp=(0.0, 0.0, 0.0), dipoles=False)
+    s = g.sanity()
+    assert s["fluid_sites"] == 720 and s["fast_sites"] == 0
+    assert (s["max_pos"][:2] == [12, 10, 6]).all()
+    assert (s["max_pos"][2] == 0).all()

@@ -236,3 +236,35 @@ def test_stress_of_an_equilibrium_field():
     s0, s1, _ = o2.stress()
     np.testing.assert_allclose(s0, 0.5 * ny * nz * 1.1 * ux * uz, rtol=1e-12)
     np.testing.assert_allclose(s1, -0.5 * ny * nz * 1.1 * ux * uz, rtol=1e-12)
+
+
+def test_sanity_full_uniform_flow_and_ties():
+    """Uniform equilibrium flow u: every site has velocity u (amass 1), so the
+    sums are ns*u, every site ties for the maximum and the reference's `>=`
+    keeps the last site of its loop, (nx, ny, nz); |u| > 0.1 makes every site
+    too fast.  One faster site then takes the maximum; a NaN is counted."""
+    from oracle.pyoracle import Oracle
+    nx, ny, nz = 5, 4, 3
+    u = (0.05, -0.07, 0.06)
+    b = boltz(2, u)
+    o = Oracle(nx, ny, nz, ncomp=2)
+    I = o.interior()
+    for c, r in ((0, 0.7), (1, 0.4)):
+        I[..., 24 * c:24 * c + 19] = r * b
+    s = o.sanity_full()
+    ns = nx * ny * nz
+    assert s["fluid_sites"] == ns and s["fast_sites"] == ns and s["nan_sites"] == 0
+    for c in (0, 1):
+        np.testing.assert_allclose(s["vel_sum"][c], ns * np.array(u), rtol=1e-12)
+        np.testing.assert_allclose(s["max_vel"][c], np.abs(u), rtol=1e-12)
+        assert (s["max_pos"][c] == [nx, ny, nz]).all()
+    np.testing.assert_allclose(s["mass"][:2], [0.7 * ns, 0.4 * ns], rtol=1e-13)
+    np.testing.assert_allclose(s["momentum"], 1.1 * ns * np.array(u), rtol=1e-12)
+    # a faster red site at (x, y, z) = (2, 3, 1), 1-based
+    I[0, 2, 1, 0:19] = 0.7 * boltz(2, (0.08, -0.02, 0.0))
+    s = o.sanity_full()
+    assert list(s["max_pos"][0][0]) == [2, 3, 1]
+    assert list(s["max_pos"][0][1]) == [nx, ny, nz] and list(s["max_pos"][1][0]) == [nx, ny, nz]
+    np.testing.assert_allclose(s["max_vel"][0][0], 0.08, rtol=1e-12)
+    I[2, 0, 4, 24 + 3] = np.nan
+    assert o.sanity_full()["nan_sites"] == 1
