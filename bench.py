@@ -41,7 +41,7 @@ def parse_plan(chosen):
     w = chosen.split()
     kind, shape, lz, stage, xg = w[1], w[2], w[3], w[4], w[5]
     tx, ty = (int(v) for v in shape.split("x"))
-    mode = 0 if kind == "two-pass" else (2 if stage == "regs" else 1)
+    mode = 0 if kind == "two-pass" else {"lds": 1, "regs": 2, "reload": 3}[stage]
     return dict(mode=mode, tx=tx, ty=ty, lz=int(lz.split("=")[1]), xg=int(xg.split("=")[1]))
 
 

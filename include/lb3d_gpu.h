@@ -136,8 +136,9 @@ int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
  * text report (may be NULL).  The lattice state is unchanged. */
 int lb3d_gpu_autotune(lb3d_ctx *ctx, int32_t trial_steps, char *report, int32_t report_len);
 /* Choose the execution plan directly: fused = 0 (two-pass), 1 (one-pass
- * kernel with tile tx x ty and z-chunk lz, look-ahead plane staged in LDS) or
- * 2 (the same with the look-ahead plane in registers); 1 and 2 binary only.
+ * kernel with tile tx x ty and z-chunk lz, look-ahead plane staged in LDS),
+ * 2 (the same with the look-ahead plane in registers) or 3 (no look-ahead
+ * copy: the collide pulls its plane again); 1-3 binary with bdist 2 only.
  * xcd_group: tiles that one XCD (and its L2) takes in a row; 1 is the plain
  * round-robin order, 0 one contiguous run of tiles per XCD.  Results do not
  * depend on the plan. */

@@ -41,10 +41,12 @@ enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_FUS, K_COUNT };
 // How a timestep is executed: the two-pass sequence (moments, [dipole],
 // collide) or, for binary lattices, the one-pass z-march kernel with a tile
 // shape (tx, ty) and a z-chunk lz.
+const char *const STAGE_NAMES[3] = {"regs", "lds", "reload"};
+
 struct Plan {
     bool fused = false;
     int tx = 64, ty = 4, lz = 0;
-    bool stage = true;  // look-ahead populations in LDS (true) or registers
+    int stage = 1;      // plane k's populations: 0 registers, 1 LDS, 2 pulled again
     uint32_t xg = 1;    // tiles per XCD chunk (xcd_tile): 1 plain order, 0 one run per XCD
 };
 
@@ -648,16 +650,16 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     if (c->p.ncomp == 2) {
         const int lzs[] = {16, 32, 64, 128, 256};
         for (int k = 0; k < N_FUSED_SHAPES; k++)
-            for (int st = 0; st < 2; st++)
+            for (int st = 0; st < 3; st++)
                 for (int lz : lzs) {
-                    if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1], st == 0)) continue;
+                    if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1], st)) continue;
                     if (lz > c->p.nz && lz != 16) continue;
                     Plan q;
                     q.fused = true;
                     q.tx = FUSED_SHAPES[k][0];
                     q.ty = FUSED_SHAPES[k][1];
                     q.lz = lz < c->p.nz ? lz : c->p.nz;
-                    q.stage = st == 0;
+                    q.stage = st;
                     q.xg = 0;
                     cands.push_back(q);
                 }
@@ -695,7 +697,7 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         char buf[112];
         std::snprintf(buf, sizeof buf, "%s %dx%d lz=%d %s xg=%u %.4f ms\n", q.fused ? "fused" : "two-pass",
                       q.fused ? q.tx : 0, q.fused ? q.ty : 0, q.fused ? q.lz : 0,
-                      q.fused ? (q.stage ? "lds" : "regs") : "-", q.xg, per);
+                      q.fused ? STAGE_NAMES[q.stage] : "-", q.xg, per);
         rep += buf;
         if (per < best) {
             best = per;
@@ -732,7 +734,7 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     char buf[112];
     std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %s xg=%u %.4f ms\n", bestp.fused ? "fused" : "two-pass",
                   bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0,
-                  bestp.fused ? (bestp.stage ? "lds" : "regs") : "-", bestp.xg, best);
+                  bestp.fused ? STAGE_NAMES[bestp.stage] : "-", bestp.xg, best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
@@ -749,7 +751,9 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     Plan q;
     q.xg = (uint32_t)xcd_group;
     q.fused = fused != 0;
-    q.stage = fused != 2;  // fused = 2: look-ahead in registers
+    // fused = 1: LDS stage, 2: registers, 3: pulled again by the collide
+    if (fused < 0 || fused > 3) return fail(LB3D_EINVAL, "fused must be 0..3");
+    q.stage = fused == 1 ? 1 : (fused == 2 ? 0 : 2);
     if (q.fused) {
         if (c->p.ncomp != 2 || c->p.bdist != 2)
             return fail(LB3D_EINVAL, "the one-pass kernel is for binary lattices with bdist 2");

@@ -345,9 +345,13 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
 // F: moments, body force, BGK relaxation towards rho*boltz(u_tilde, du),
 // code/lb3d_collision.F90:188-289 with lb3d_get_sc_u_eq (:560-587) and
 // lb3d_get_sc_du (:523-556).  Writes the post-collision populations of site s.
-template <typename R, bool Dev, int C, int BD>
-__device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
-                                         int x, int y, int z, uint32_t s, R *__restrict__ fout)
+// get(c, i) returns population i of component c; with Refetch the moments
+// pass and the relaxation pass each call it anew (a compiler memory barrier
+// between them keeps the loads from being shared), so that no population
+// stays live in registers across the two passes.
+template <typename R, bool Dev, int C, int BD, bool Refetch = false, typename Get>
+__device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][3], int x, int y, int z,
+                                           uint32_t s, R *__restrict__ fout)
 {
     R rho[C], mom[C][3];
 #pragma unroll
@@ -359,7 +363,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
     for (int i = 0; i < 19; i++) {
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            R v = n[c][i];
+            R v = get(c, i);
             rho[c] = rho[c] + v * R(GD[i]);
             R ng = v * R(GD[i]);
             if (CX[i] != 0) mom[c][0] = mom[c][0] + ng * R(CX[i]);
@@ -367,6 +371,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
             if (CZ[i] != 0) mom[c][2] = mom[c][2] + ng * R(CZ[i]);
         }
     }
+    if constexpr (Refetch) asm volatile("" ::: "memory");
     R rhof[C];  // full densities
 #pragma unroll
     for (int c = 0; c < C; c++) rhof[c] = Dev ? P.rho0[c] + rho[c] : rho[c];
@@ -410,19 +415,27 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
                 // deviation of rho*b_i from w_i*rho0: w_i*drho + rho*(b_i - w_i)
                 R wi = i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0);
                 R deq = wi * rho[c] + rhof[c] * b[i];
-                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s,
-                         n[c][i] - P.omega[c] * (n[c][i] - deq));
+                R v = get(c, i);
+                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - deq));
             }
         } else {
             boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];
-                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s,
-                         n[c][i] - P.omega[c] * (n[c][i] - neq));
+                R v = get(c, i);
+                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - neq));
             }
         }
     }
+}
+
+template <typename R, bool Dev, int C, int BD>
+__device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
+                                         int x, int y, int z, uint32_t s, R *__restrict__ fout)
+{
+    bgk_site_g<R, Dev, C, BD>(P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, F, x, y,
+                              z, s, fout);
 }
 
 // --------------------------------------------------------------------------
@@ -465,7 +478,7 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
 // k_fused2: the whole binary timestep in one pass.  A workgroup owns a TX x TY
 // column tile and marches along z through [z0, z0+lz).  Each thread pulls the
 // streamed populations of its site one plane ahead (plane k+1) and keeps them
-// in registers while it collides plane k; the densities psi of plane k+1 for
+// in registers (or LDS) while it collides plane k; the densities psi of plane k+1 for
 // the tile and its one-site halo go into an LDS ring of three planes, the
 // halo sites by a sum-only pull.  Nothing but the populations is read from or
 // written to memory (Lees-Edwards halo columns read face_pc / face_aux).
@@ -496,7 +509,12 @@ __device__ __forceinline__ void psi_at(const Phys<R> &P, const R *__restrict__ f
     p1 = psi_of<Dev>(r1, P.emr0[1]);
 }
 
-template <typename R, bool Dev, bool LE, int TX, int TY, bool Stage>
+// SM selects where the streamed populations of plane k wait for the collide:
+// 0 registers (pulled one plane ahead), 1 an LDS stage (pulled one plane
+// ahead), 2 nowhere: the collide pulls them again, once for the moments and
+// once for the relaxation (L2 hits one plane after the psi pass read them),
+// which leaves the fewest live registers.
+template <typename R, bool Dev, bool LE, int TX, int TY, int SM>
 __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restrict__ f,
                                                     const R *__restrict__ face_pc,
                                                     const R *__restrict__ face_aux,
@@ -505,8 +523,8 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     constexpr int T = TX * TY, LX = TX + 2, LY = TY + 2, LP = LX * LY;
     constexpr int NH = 2 * LX + 2 * TY;  // halo positions per plane
     __shared__ R ring[3][2][LP];         // psi of planes k-1, k, k+1 (tile + halo)
-    // populations of plane k+1: in LDS, one column per thread (Stage), or in registers
-    __shared__ R stage[Stage ? 38 : 1][Stage ? T : 1];
+    // populations of plane k+1: in LDS, one column per thread (SM 1), or in registers (SM 0)
+    __shared__ R stage[SM == 1 ? 38 : 1][SM == 1 ? T : 1];
     R nxt[2][19];
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = tid / TX;
@@ -555,10 +573,10 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             for (int i = 0; i < 19; i++) {
                 R v0 = pull<R, 2, LE>(f, face_pc, P, 0, i, xo, yo, zs);
                 R v1 = pull<R, 2, LE>(f, face_pc, P, 1, i, xo, yo, zs);
-                if constexpr (Stage) {
+                if constexpr (SM == 1) {
                     stage[i][tid] = v0;
                     stage[19 + i][tid] = v1;
-                } else {
+                } else if constexpr (SM == 0) {
                     nxt[0][i] = v0;
                     nxt[1][i] = v1;
                 }
@@ -591,14 +609,16 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     for (int k = z0; k < zend; k++) {
         // the populations of plane k from the stage (own column only)
         R cur[2][19];
+        if constexpr (SM != 2) {
 #pragma unroll
-        for (int i = 0; i < 19; i++) {
-            if constexpr (Stage) {
-                cur[0][i] = stage[i][tid];
-                cur[1][i] = stage[19 + i][tid];
-            } else {
-                cur[0][i] = nxt[0][i];
-                cur[1][i] = nxt[1][i];
+            for (int i = 0; i < 19; i++) {
+                if constexpr (SM == 1) {
+                    cur[0][i] = stage[i][tid];
+                    cur[1][i] = stage[19 + i][tid];
+                } else {
+                    cur[0][i] = nxt[0][i];
+                    cur[1][i] = nxt[1][i];
+                }
             }
         }
         fill(k + 1, sp);
@@ -629,7 +649,15 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             F[0][0] = (-(P.g_br * frb0)) * psr;
             F[0][1] = (-(P.g_br * frb1)) * psr;
             F[0][2] = (-(P.g_br * frb2)) * psr;
-            bgk_site<R, Dev, 2, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+            if constexpr (SM == 2)
+                bgk_site_g<R, Dev, 2, 2, true>(
+                    P,
+                    [&](int c, int i) __attribute__((always_inline)) {
+                        return pull<R, 2, LE>(f, face_pc, P, c, i, x, y, k);
+                    },
+                    F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+            else
+                bgk_site<R, Dev, 2, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
         }
         __syncthreads();
         int t = sm;

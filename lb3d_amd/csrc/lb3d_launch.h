@@ -294,33 +294,41 @@ class LauncherT : public Launcher {
     {
         return (38 * TX * TY + 6 * (TX + 2) * (TY + 2)) * sizeof(R) <= 96 * 1024;
     }
+    // stage: 0 registers, 1 LDS, 2 reload (k_fused2's SM)
     template <int TX, int TY>
-    bool fused_t(const DevBufs &b, int lz, bool stage, hipStream_t s, bool launch)
+    bool fused_t(const DevBufs &b, int lz, int stage, hipStream_t s, bool launch)
     {
-        if (stage) {
+        if (stage == 1) {
             if constexpr (!fits<TX, TY>()) {
                 return false;
             } else {
-                if (launch) fused_go<TX, TY, true>(b, lz, s);
+                if (launch) fused_go<TX, TY, 1>(b, lz, s);
                 return true;
             }
         }
-        if (launch) fused_go<TX, TY, false>(b, lz, s);
-        return true;
+        if (stage == 0) {
+            if (launch) fused_go<TX, TY, 0>(b, lz, s);
+            return true;
+        }
+        if (stage == 2) {
+            if (launch) fused_go<TX, TY, 2>(b, lz, s);
+            return true;
+        }
+        return false;
     }
-    template <int TX, int TY, bool ST>
+    template <int TX, int TY, int SM>
     void fused_go(const DevBufs &b, int lz, hipStream_t s)
     {
         unsigned grid = xcd_round(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY) * ((P.nz + lz - 1) / lz));
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (le)
-            k_fused2<R, Dev, true, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
+            k_fused2<R, Dev, true, TX, TY, SM><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.face_pc,
                                                                          (const R *)b.face_aux, fo, lz, xg);
         else
-            k_fused2<R, Dev, false, TX, TY, ST><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz, xg);
+            k_fused2<R, Dev, false, TX, TY, SM><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz, xg);
     }
-    bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, bool st, hipStream_t s, bool launch)
+    bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, int st, hipStream_t s, bool launch)
     {
         if (C != 2 || lz < 1 || P.bdist != 2) return false;
         if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, st, s, launch);
@@ -331,11 +339,11 @@ class LauncherT : public Launcher {
         if (tx == 32 && ty == 4) return fused_t<32, 4>(b, lz, st, s, launch);
         return false;
     }
-    bool fused(const DevBufs &b, int tx, int ty, int lz, bool stage, hipStream_t s) override
+    bool fused(const DevBufs &b, int tx, int ty, int lz, int stage, hipStream_t s) override
     {
         return fused_dispatch(b, tx, ty, lz, stage, s, true);
     }
-    bool fused_supported(int tx, int ty, bool stage) override
+    bool fused_supported(int tx, int ty, int stage) override
     {
         DevBufs none;
         return fused_dispatch(none, tx, ty, 1, stage, nullptr, false);

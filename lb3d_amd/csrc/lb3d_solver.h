@@ -33,9 +33,10 @@ class Launcher {
     virtual void le_faces(const DevBufs &b, bool post, hipStream_t s) = 0;
     // the whole binary step in one kernel (f[cur] -> f[1-cur]); false if the
     // tile shape has no instantiation or the form is not binary
-    // stage: the look-ahead plane's populations in LDS (true) or registers
-    virtual bool fused(const DevBufs &b, int tx, int ty, int lz, bool stage, hipStream_t s) = 0;
-    virtual bool fused_supported(int tx, int ty, bool stage) = 0;
+    // stage: where plane k's populations wait for the collide: 0 registers,
+    // 1 LDS (both pulled one plane ahead), 2 pulled again by the collide
+    virtual bool fused(const DevBufs &b, int tx, int ty, int lz, int stage, hipStream_t s) = 0;
+    virtual bool fused_supported(int tx, int ty, int stage) = 0;
     virtual void init_noise(const DevBufs &b, uint64_t seed, const double rho_mean[3],
                             const double amp[3], int dipoles, hipStream_t s) = 0;
     // per-block FP64 partial sums for the sanity reductions (NDIAG per block)

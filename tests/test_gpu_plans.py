@@ -14,10 +14,11 @@ SHAPES = [(64, 4), (32, 8), (64, 8), (32, 16), (128, 4), (32, 4)]
 
 def fused_shapes(precision):
     """(mode, tx, ty) of the one-pass kernel that exist for this form:
-    mode 1 stages the look-ahead plane in LDS, mode 2 keeps it in registers."""
+    mode 1 stages the look-ahead plane in LDS, mode 2 keeps it in registers,
+    mode 3 pulls plane k again in the collide."""
     g = Lattice(8, 8, 8, ncomp=2, precision=precision)
     ok = []
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         for tx, ty in SHAPES:
             try:
                 g.set_plan(mode, tx, ty, 4)
