@@ -107,6 +107,7 @@ contains
     else
        p%rho0 = [fr, fb, fg]
     end if
+    where (rho0 > 0.0d0) p%rho0 = rho0
     where (p%rho0 <= 0.0_c_double) p%rho0 = 0.7_c_double
 
     call check(lb3d_gpu_create(p, ctx), 'lb3d_gpu_create')

@@ -66,7 +66,10 @@ module lb3d_host_config
   character(len=16) :: precision = 'fp64'
   logical :: autotune = .false.
   logical :: fast_loop = .true.         ! replay captured steps between I/O points
-  namelist /gpu_input/ ncomp, precision, autotune, fast_loop
+  ! reference densities of the FP32 deviation form; <= 0: the mean of the
+  ! initial (or restored) state.  Fixing them makes FP32 restarts bitwise.
+  real(8) :: rho0(3) = (/0.0d0, 0.0d0, 0.0d0/)
+  namelist /gpu_input/ ncomp, precision, autotune, fast_loop, rho0
 
 contains
 

@@ -74,10 +74,12 @@ def test_fortran_host_staged_loop_equals_graph_loop(tmp_path):
                                                   ("leesEdwards", 2, "fp32")])
 def test_fortran_host_restart_equivalence(tmp_path, name, ncomp, precision):
     """10 steps straight against 5 steps, the host's own XDR checkpoint (with
-    checktopo and shear_sum), a restore and 5 more steps: identical dumps."""
+    checktopo and shear_sum), a restore and 5 more steps: identical dumps.
+    The FP32 form keeps its deviation base fixed (&gpu_input rho0) so that
+    the restored deviations are the same numbers."""
     gold = np.load(GOLD)
     base = open(os.path.join(INPUTS, f"{name}.in")).read()
-    extra = f"\n&gpu_input\nprecision = '{precision}'\n/\n"
+    extra = f"\n&gpu_input\nprecision = '{precision}'\nrho0 = 0.35, 0.35, 0.35\n/\n"
 
     def run(d, text):
         d.mkdir(parents=True, exist_ok=True)
