@@ -128,6 +128,10 @@ int lb3d_gpu_collision(lb3d_ctx *ctx);                           /* code/lb3d_co
 /* nsteps whole timesteps nt+1 .. nt+nsteps (the body of lb3d_time_loop,
  * code/lb3d_time_loop.F90:58-92), replayed from a captured HIP graph. */
 int lb3d_gpu_run_steps(lb3d_ctx *ctx, int32_t nsteps);
+/* Timing support (not a reference routine): lb3d_gpu_run_steps between two
+ * HIP events recorded on the context's stream; *ms is the elapsed time
+ * between them.  Synchronises. */
+int lb3d_gpu_run_steps_timed(lb3d_ctx *ctx, int32_t nsteps, double *ms);
 
 /* Autotuner (not a reference routine): times every execution plan of this
  * lattice -- the two-pass kernels and, for binary lattices, the one-pass

@@ -633,6 +633,31 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     return LB3D_OK;
 }
 
+int lb3d_gpu_run_steps_timed(lb3d_ctx *c, int32_t nsteps, double *ms)
+{
+    g_err.clear();
+    if (!c || !ms) return fail(LB3D_EINVAL, "NULL argument");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "run_steps needs a completed timestep");
+    int rc = sync_le_counter(c);
+    if (rc) return rc;
+    if ((rc = ensure_graph(c))) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    rc = lb3d_gpu_run_steps(c, nsteps);
+    if (rc == LB3D_OK) {
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 // Autotuner: times each candidate plan on the context's own lattice.  Trial
 // steps read f[cur] and write f[1-cur] without swapping, so the state is
 // unchanged (the LE scalars are not advanced; the ternary dipole field is

@@ -32,7 +32,7 @@ EXPORTS = [
     "lb3d_gpu_abi_version", "lb3d_gpu_last_error", "lb3d_gpu_create", "lb3d_gpu_destroy",
     "lb3d_gpu_upload_sites", "lb3d_gpu_download_sites", "lb3d_gpu_init_noise",
     "lb3d_gpu_bc_init", "lb3d_gpu_force_apply", "lb3d_gpu_bc_apply", "lb3d_gpu_advection",
-    "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
+    "lb3d_gpu_collision", "lb3d_gpu_run_steps", "lb3d_gpu_run_steps_timed", "lb3d_gpu_get_nt", "lb3d_gpu_diagnostics",
     "lb3d_gpu_le_state", "lb3d_gpu_get_populations", "lb3d_gpu_profile_steps",
     "lb3d_gpu_get_density", "lb3d_gpu_get_momentum", "lb3d_gpu_le_halo", "lb3d_gpu_sanity",
     "lb3d_gpu_batch_sanity",
@@ -108,6 +108,7 @@ def lib():
         L.lb3d_gpu_advection.argtypes = [vp]
         L.lb3d_gpu_collision.argtypes = [vp]
         L.lb3d_gpu_run_steps.argtypes = [vp, ip]
+        L.lb3d_gpu_run_steps_timed.argtypes = [vp, ip, dp]
         L.lb3d_gpu_get_nt.argtypes = [vp, ctypes.POINTER(ip)]
         L.lb3d_gpu_diagnostics.argtypes = [vp, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.lb3d_gpu_le_state.argtypes = [vp, dp, ctypes.POINTER(ip), dp, dp]
@@ -336,6 +337,12 @@ class Lattice:
     def time_loop(self, nsteps):
         """nsteps timesteps replayed from a captured HIP graph."""
         check(lib().lb3d_gpu_run_steps(self._h, int(nsteps)), "run_steps")
+
+    def time_loop_timed(self, nsteps):
+        """time_loop between HIP events on the library's stream; returns ms."""
+        ms = ctypes.c_double()
+        check(lib().lb3d_gpu_run_steps_timed(self._h, int(nsteps), ctypes.byref(ms)), "run_steps_timed")
+        return ms.value
 
     def sanity_check(self):
         m = (ctypes.c_double * 3)()
