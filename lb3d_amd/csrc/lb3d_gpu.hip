@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -239,7 +240,8 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
             return cleanup(LB3D_ENOMEM, std::string("hipMalloc populations: ") + hipGetErrorString(e));
     if ((e = hipMalloc(&c->b.aux, c->aux_bytes)) != hipSuccess)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc aux: ") + hipGetErrorString(e));
-    if (c->db_bytes && (e = hipMalloc(&c->b.db, c->db_bytes)) != hipSuccess)
+    if (c->db_bytes && ((e = hipMalloc(&c->b.db, c->db_bytes)) != hipSuccess ||
+                        (e = hipMalloc(&c->b.db_alt, c->db_bytes)) != hipSuccess))
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc db: ") + hipGetErrorString(e));
     if (c->le) {
         if ((e = hipMalloc(&c->b.face_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
@@ -276,6 +278,7 @@ int lb3d_gpu_destroy(lb3d_ctx *c)
         if (c->b.f[k]) (void)hipFree(c->b.f[k]);
     if (c->b.aux) (void)hipFree(c->b.aux);
     if (c->b.db) (void)hipFree(c->b.db);
+    if (c->b.db_alt) (void)hipFree(c->b.db_alt);
     if (c->b.face_pc) (void)hipFree(c->b.face_pc);
     if (c->b.face_aux) (void)hipFree(c->b.face_aux);
     if (c->b.le) (void)hipFree(c->b.le);
@@ -383,9 +386,11 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
     if (!c->aux_valid && c->aux_from_prev) {
         // the one-pass plan keeps no moments: recompute those of the last
         // advection from the previous post-collision populations f[1-cur]
-        c->b.cur = 1 - c->b.cur;
-        c->L->moments(c->b, c->stream);
-        c->b.cur = 1 - c->b.cur;
+        // (ternary: with the dipole field of that step, now in db_alt)
+        DevBufs prev = c->b;
+        prev.cur = 1 - c->b.cur;
+        if (C == 3) prev.db = c->b.db_alt;
+        c->L->moments(prev, c->stream);
         HIP_TRY(hipGetLastError());
         c->aux_valid = true;
         c->aux_from_prev = false;
@@ -528,11 +533,14 @@ int lb3d_gpu_advection(lb3d_ctx *c)
     return LB3D_OK;
 }
 
-static void launch_collision(lb3d_ctx *c)
+// commit = false (autotune trials): the ternary one-pass kernel's new dipole
+// field stays in db_alt and db keeps the state
+static void launch_collision(lb3d_ctx *c, bool commit = true)
 {
     if (c->plan.fused) {
         // plans are validated when set (lb3d_gpu_set_plan, autotune, create)
         if (!c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream)) abort();
+        if (c->p.ncomp == 3 && commit) std::swap(c->b.db, c->b.db_alt);
     } else {
         c->L->dipole(c->b, c->stream);
         c->L->collide(c->b, c->stream);
@@ -672,6 +680,22 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
     Plan two;
     two.fused = false;
     cands.push_back(two);
+    if (c->p.ncomp == 3) {
+        const int lzs[] = {8, 16, 32, 64};
+        for (int k = 0; k < N_FUSED_SHAPES; k++)
+            for (int lz : lzs) {
+                if (!c->L->fused_supported(FUSED_SHAPES[k][0], FUSED_SHAPES[k][1], 2)) continue;
+                if (lz > c->p.nz && lz != 8) continue;
+                Plan q;
+                q.fused = true;
+                q.tx = FUSED_SHAPES[k][0];
+                q.ty = FUSED_SHAPES[k][1];
+                q.lz = lz < c->p.nz ? lz : c->p.nz;
+                q.stage = 2;
+                q.xg = 0;
+                cands.push_back(q);
+            }
+    }
     if (c->p.ncomp == 2) {
         const int lzs[] = {16, 32, 64, 128, 256};
         for (int k = 0; k < N_FUSED_SHAPES; k++)
@@ -708,7 +732,7 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
             if (c->le) c->L->le_faces(c->b, false, c->stream);
             if (!q.fused) c->L->moments(c->b, c->stream);
             if (c->le) c->L->le_faces(c->b, true, c->stream);
-            launch_collision(c);
+            launch_collision(c, false);
         };
         trial();  // warm-up
         HIP_TRY(hipEventRecord(e0, c->stream));
@@ -780,8 +804,10 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     if (fused < 0 || fused > 3) return fail(LB3D_EINVAL, "fused must be 0..3");
     q.stage = fused == 1 ? 1 : (fused == 2 ? 0 : 2);
     if (q.fused) {
-        if (c->p.ncomp != 2 || c->p.bdist != 2)
-            return fail(LB3D_EINVAL, "the one-pass kernel is for binary lattices with bdist 2");
+        if (c->p.ncomp == 2 && c->p.bdist != 2)
+            return fail(LB3D_EINVAL, "the binary one-pass kernel needs bdist 2");
+        if (c->p.ncomp == 3 && (fused != 3 || c->le))
+            return fail(LB3D_EINVAL, "the ternary one-pass kernel is fused = 3 on periodic lattices");
         bool ok = c->L->fused_supported(tx, ty, q.stage);
         if (!ok || lz < 1) return fail(LB3D_EINVAL, "no such tile shape");
         q.tx = tx;

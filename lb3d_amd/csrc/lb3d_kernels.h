@@ -135,14 +135,11 @@ __device__ __forceinline__ R psi_of(R rho, R emr0)
 // da = sum_i g_i n_s,i(x-c_i) db(x-c_i) (:130-148).  The streamed
 // populations themselves are not stored: k_collide pulls them again.
 template <typename R, bool Dev, int C, bool LE>
-__device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
+__device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
-                                             const R *__restrict__ db, R *__restrict__ aux,
-                                             uint32_t s)
+                                             const R *__restrict__ db, int x, int y, int z, R (&rho)[C],
+                                             R (&da)[3])
 {
-    int x, y, z;
-    site_xyz(s, P.nx, P.ny, x, y, z);
-    R rho[C];
 #pragma unroll
     for (int c = 0; c < C; c++) rho[c] = R(0);
     R da0 = R(0), da1 = R(0), da2 = R(0);
@@ -166,15 +163,30 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
             }
         }
     }
+    da[0] = da0;
+    da[1] = da1;
+    da[2] = da2;
+}
+
+template <typename R, bool Dev, int C, bool LE>
+__device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
+                                             const R *__restrict__ face_pc,
+                                             const R *__restrict__ db, R *__restrict__ aux,
+                                             uint32_t s)
+{
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R rho[C], da[3];
+    moments_core<R, Dev, C, LE>(P, f, face_pc, db, x, y, z, rho, da);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
         st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
     }
     if constexpr (C == 3) {
-        st_plane(aux + (size_t)6 * P.ns, s, da0);
-        st_plane(aux + (size_t)7 * P.ns, s, da1);
-        st_plane(aux + (size_t)8 * P.ns, s, da2);
+        st_plane(aux + (size_t)6 * P.ns, s, da[0]);
+        st_plane(aux + (size_t)7 * P.ns, s, da[1]);
+        st_plane(aux + (size_t)8 * P.ns, s, da[2]);
     }
 }
 
@@ -191,33 +203,26 @@ __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict_
 
 // --------------------------------------------------------------------------
 // k_dipole: dipole relaxation, code/lb3d_collision.F90:296-403 (ternary).
-template <typename R, bool Dev>
-__device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restrict__ aux,
-                                            R *__restrict__ db, uint32_t s)
+// The dipole update of one site from its own advected dipole hs = da(x) and
+// surfactant density, and per neighbour i (x + c_i) get_drb(i) = rho_b - rho_r
+// and get_da(i, a).  Writes the new db(x) to out.
+template <typename R, bool Dev, typename GetDrb, typename GetDa>
+__device__ __forceinline__ void dipole_core(const Phys<R> &P, R hs0, R hs1, R hs2, R rho_s, GetDrb get_drb,
+                                            GetDa get_da, R (&out)[3])
 {
-    int x, y, z;
-    site_xyz(s, P.nx, P.ny, x, y, z);
-    R hs0 = ld_plane(plane_of(aux, 6u, P.ns), s), hs1 = ld_plane(plane_of(aux, 7u, P.ns), s),
-      hs2 = ld_plane(plane_of(aux, 8u, P.ns), s);
-    R rho_s = ld_plane(plane_of(aux, 5u, P.ns), s);
     if constexpr (Dev) rho_s = P.rho0[2] + rho_s;
     R sN = rho_s > R(CLAMP_SURF) ? rho_s : R(CLAMP_SURF);
     R sd0 = hs0 / sN, sd1 = hs1 / sN, sd2 = hs2 / sN;
     R hc0 = R(0), hc1 = R(0), hc2 = R(0);
 #pragma unroll
     for (int i = 0; i < 18; i++) {
-        int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
         // -sign(1,g_bs)*(rho_b - rho_r)(x+c_i)*g_i (raw densities, :361-369)
-        R rr = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
-        R rb = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
         R gs = R(GD[i]);
-        R hc_m = -((P.sign_g_bs * (rb - rr)) * gs);
+        R hc_m = -((P.sign_g_bs * get_drb(i)) * gs);
         if (CX[i] != 0) hc0 = hc0 + hc_m * R(CX[i]);
         if (CY[i] != 0) hc1 = hc1 + hc_m * R(CY[i]);
         if (CZ[i] != 0) hc2 = hc2 + hc_m * R(CZ[i]);
-        R p0 = aux_at<R, false>(aux, nullptr, P, 6, xa, ya, za);
-        R p1 = aux_at<R, false>(aux, nullptr, P, 7, xa, ya, za);
-        R p2 = aux_at<R, false>(aux, nullptr, P, 8, xa, ya, za);
+        R p0 = get_da(i, 0), p1 = get_da(i, 1), p2 = get_da(i, 2);
         R hs_dot = R(2.0) * (R(CX[i]) * p0 + R(CY[i]) * p1 + R(CZ[i]) * p2);
         hs0 = hs0 + (p0 - R(CX[i]) * hs_dot) * gs;
         hs1 = hs1 + (p1 - R(CY[i]) * hs_dot) * gs;
@@ -236,9 +241,34 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
     e0 = e0 * P.d_0;
     e1 = e1 * P.d_0;
     e2 = e2 * P.d_0;
-    st_plane(db, s, P.omega_d * e0 + P.one_m_omega_d * sd0);
-    st_plane(db + (size_t)P.ns, s, P.omega_d * e1 + P.one_m_omega_d * sd1);
-    st_plane(db + (size_t)2 * P.ns, s, P.omega_d * e2 + P.one_m_omega_d * sd2);
+    out[0] = P.omega_d * e0 + P.one_m_omega_d * sd0;
+    out[1] = P.omega_d * e1 + P.one_m_omega_d * sd1;
+    out[2] = P.omega_d * e2 + P.one_m_omega_d * sd2;
+}
+
+template <typename R, bool Dev>
+__device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restrict__ aux,
+                                            R *__restrict__ db, uint32_t s)
+{
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R out[3];
+    dipole_core<R, Dev>(
+        P, ld_plane(plane_of(aux, 6u, P.ns), s), ld_plane(plane_of(aux, 7u, P.ns), s),
+        ld_plane(plane_of(aux, 8u, P.ns), s), ld_plane(plane_of(aux, 5u, P.ns), s),
+        [&](int i) __attribute__((always_inline)) {
+            int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
+            R rr = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
+            R rb = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
+            return rb - rr;
+        },
+        [&](int i, int a) __attribute__((always_inline)) {
+            return aux_at<R, false>(aux, nullptr, P, 6 + a, x + CX[i], y + CY[i], z + CZ[i]);
+        },
+        out);
+    st_plane(db, s, out[0]);
+    st_plane(db + (size_t)P.ns, s, out[1]);
+    st_plane(db + (size_t)2 * P.ns, s, out[2]);
 }
 
 template <typename R, bool Dev>
@@ -248,6 +278,67 @@ __global__ __launch_bounds__(256) void k_dipole(Phys<R> P, const R *__restrict__
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     dipole_site<R, Dev>(P, aux, db, s);
+}
+
+// Ternary Shan-Chen forces (code/lb3d_force_shanchen.F90:215-330) of one site
+// from its dipole dbx and psi ps_own (stored form), and per neighbour i
+// get_psi(i, c) (stored form) and get_db(i, a).
+template <typename R, bool Dev, typename GetPsi, typename GetDb>
+__device__ __forceinline__ void sc_forces3(const Phys<R> &P, const R (&dbx)[3], const R (&ps_own)[3],
+                                           GetPsi get_psi, GetDb get_db, R F[3][3])
+{
+    R frb[3] = {R(0), R(0), R(0)}, fbr[3] = {R(0), R(0), R(0)};
+    R fcs[3] = {R(0), R(0), R(0)}, fsc[3] = {R(0), R(0), R(0)}, fss[3] = {R(0), R(0), R(0)};
+    const R dbx0 = dbx[0], dbx1 = dbx[1], dbx2 = dbx[2];
+    const R dpsi_br = Dev ? P.psi0[1] - P.psi0[0] : R(0);
+#pragma unroll
+    for (int i = 0; i < 18; i++) {
+        const R c0 = R(CX[i]), c1 = R(CY[i]), c2 = R(CZ[i]);
+        const R gs = R(GD[i]);
+        R pr = get_psi(i, 0);
+        R pb = get_psi(i, 1);
+        R ps = get_psi(i, 2);
+        if constexpr (Dev) ps = P.psi0[2] + ps;
+        R green_r = pb * gs, green_b = pr * gs;
+        if (CX[i] != 0) { frb[0] = frb[0] + green_r * c0; fbr[0] = fbr[0] + green_b * c0; }
+        if (CY[i] != 0) { frb[1] = frb[1] + green_r * c1; fbr[1] = fbr[1] + green_b * c1; }
+        if (CZ[i] != 0) { frb[2] = frb[2] + green_r * c2; fbr[2] = fbr[2] + green_b * c2; }
+        R d0 = get_db(i, 0) * ps;
+        R d1 = get_db(i, 1) * ps;
+        R d2 = get_db(i, 2) * ps;
+        R d_dotc = d0 * c0 + d1 * c1 + d2 * c2;
+        fcs[0] = fcs[0] + (d0 - (R(2.0) * d_dotc) * c0) * gs;
+        fcs[1] = fcs[1] + (d1 - (R(2.0) * d_dotc) * c1) * gs;
+        fcs[2] = fcs[2] + (d2 - (R(2.0) * d_dotc) * c2) * gs;
+        R di_dotc = dbx0 * c0 + dbx1 * c1 + dbx2 * c2;
+        R greenc_s = ((pb - pr) + dpsi_br) * gs;
+        fsc[0] = fsc[0] + greenc_s * (dbx0 - (R(2.0) * di_dotc) * c0);
+        fsc[1] = fsc[1] + greenc_s * (dbx1 - (R(2.0) * di_dotc) * c1);
+        fsc[2] = fsc[2] + greenc_s * (dbx2 - (R(2.0) * di_dotc) * c2);
+        R di_dot_d = d0 * dbx0 + d1 * dbx1 + d2 * dbx2;
+        R green_ss = R(8.0) * gs;
+        R a = di_dot_d - (R(2.0) * di_dotc) * d_dotc;
+        fss[0] = fss[0] + green_ss * ((a * c0 + d0 * di_dotc) + d_dotc * dbx0);
+        fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
+        fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
+    }
+    R psr = ps_own[0], psb = ps_own[1], pss = ps_own[2];
+    if constexpr (Dev) {
+        psr = P.psi0[0] + psr;
+        psb = P.psi0[1] + psb;
+        pss = P.psi0[2] + pss;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        R f_br = -(P.g_br * fbr[k]);
+        R f_rb = -(P.g_br * frb[k]);
+        R f_sc = P.two_g_bs * fsc[k];
+        R f_cs = -(P.two_g_bs * fcs[k]);
+        R f_ss = -(P.g_ss * fss[k]);
+        F[1][k] = (f_br + f_cs) * psb;
+        F[0][k] = (f_rb - f_cs) * psr;
+        F[2][k] = (f_sc + f_ss) * pss;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -283,60 +374,18 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             F[0][k] = (-(P.g_br * frb[k])) * psr;
         }
     } else {
-        R fcs[3] = {R(0), R(0), R(0)}, fsc[3] = {R(0), R(0), R(0)}, fss[3] = {R(0), R(0), R(0)};
-        R dbx0 = ld_plane(db, s), dbx1 = ld_plane(plane_of(db, 1u, P.ns), s),
-          dbx2 = ld_plane(plane_of(db, 2u, P.ns), s);
-        const R dpsi_br = Dev ? P.psi0[1] - P.psi0[0] : R(0);
-#pragma unroll
-        for (int i = 0; i < 18; i++) {
-            int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
-            const R c0 = R(CX[i]), c1 = R(CY[i]), c2 = R(CZ[i]);
-            const R gs = R(GD[i]);
-            R pr = aux_at<R, false>(aux, nullptr, P, 0, xa, ya, za);
-            R pb = aux_at<R, false>(aux, nullptr, P, 1, xa, ya, za);
-            R ps = aux_at<R, false>(aux, nullptr, P, 2, xa, ya, za);
-            if constexpr (Dev) ps = P.psi0[2] + ps;
-            R green_r = pb * gs, green_b = pr * gs;
-            if (CX[i] != 0) { frb[0] = frb[0] + green_r * c0; fbr[0] = fbr[0] + green_b * c0; }
-            if (CY[i] != 0) { frb[1] = frb[1] + green_r * c1; fbr[1] = fbr[1] + green_b * c1; }
-            if (CZ[i] != 0) { frb[2] = frb[2] + green_r * c2; fbr[2] = fbr[2] + green_b * c2; }
-            R d0 = aux_at<R, false>(db, nullptr, P, 0, xa, ya, za) * ps;
-            R d1 = aux_at<R, false>(db, nullptr, P, 1, xa, ya, za) * ps;
-            R d2 = aux_at<R, false>(db, nullptr, P, 2, xa, ya, za) * ps;
-            R d_dotc = d0 * c0 + d1 * c1 + d2 * c2;
-            fcs[0] = fcs[0] + (d0 - (R(2.0) * d_dotc) * c0) * gs;
-            fcs[1] = fcs[1] + (d1 - (R(2.0) * d_dotc) * c1) * gs;
-            fcs[2] = fcs[2] + (d2 - (R(2.0) * d_dotc) * c2) * gs;
-            R di_dotc = dbx0 * c0 + dbx1 * c1 + dbx2 * c2;
-            R greenc_s = ((pb - pr) + dpsi_br) * gs;
-            fsc[0] = fsc[0] + greenc_s * (dbx0 - (R(2.0) * di_dotc) * c0);
-            fsc[1] = fsc[1] + greenc_s * (dbx1 - (R(2.0) * di_dotc) * c1);
-            fsc[2] = fsc[2] + greenc_s * (dbx2 - (R(2.0) * di_dotc) * c2);
-            R di_dot_d = d0 * dbx0 + d1 * dbx1 + d2 * dbx2;
-            R green_ss = R(8.0) * gs;
-            R a = di_dot_d - (R(2.0) * di_dotc) * d_dotc;
-            fss[0] = fss[0] + green_ss * ((a * c0 + d0 * di_dotc) + d_dotc * dbx0);
-            fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
-            fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
-        }
-        R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s),
-          pss = ld_plane(plane_of(aux, 2u, P.ns), s);
-        if constexpr (Dev) {
-            psr = P.psi0[0] + psr;
-            psb = P.psi0[1] + psb;
-            pss = P.psi0[2] + pss;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            R f_br = -(P.g_br * fbr[k]);
-            R f_rb = -(P.g_br * frb[k]);
-            R f_sc = P.two_g_bs * fsc[k];
-            R f_cs = -(P.two_g_bs * fcs[k]);
-            R f_ss = -(P.g_ss * fss[k]);
-            F[1][k] = (f_br + f_cs) * psb;
-            F[0][k] = (f_rb - f_cs) * psr;
-            F[2][k] = (f_sc + f_ss) * pss;
-        }
+        const R dbx[3] = {ld_plane(db, s), ld_plane(plane_of(db, 1u, P.ns), s), ld_plane(plane_of(db, 2u, P.ns), s)};
+        const R ps_own[3] = {ld_plane(aux, s), ld_plane(plane_of(aux, 1u, P.ns), s),
+                             ld_plane(plane_of(aux, 2u, P.ns), s)};
+        sc_forces3<R, Dev>(
+            P, dbx, ps_own,
+            [&](int i, int c) __attribute__((always_inline)) {
+                return aux_at<R, false>(aux, nullptr, P, c, x + CX[i], y + CY[i], z + CZ[i]);
+            },
+            [&](int i, int a) __attribute__((always_inline)) {
+                return aux_at<R, false>(db, nullptr, P, a, x + CX[i], y + CY[i], z + CZ[i]);
+            },
+            F);
     }
 }
 
@@ -665,6 +714,159 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
         s0 = sp;
         sp = t;
     }
+}
+
+// --------------------------------------------------------------------------
+// The whole ternary timestep in one pass (periodic lattices): advection and
+// moments (code/lb3d_advection.F90:54-151), dipole relaxation
+// (code/lb3d_collision.F90:296-403), ternary Shan-Chen forces
+// (code/lb3d_force_shanchen.F90:215-330) and BGK, marching along z through a
+// TX x TY column tile.  The force at a site needs psi and the new dipole
+// field one site around it, and the new dipole field needs the moments one
+// site further out, so per plane the work-group computes
+//   the moments of plane k+2 on the tile + 2 sites (LDS ring of 4 planes),
+//   the dipole field of plane k+1 on the tile + 1 site (LDS ring of 3 planes),
+//   and collides plane k (pulling its populations again, an L2 hit).
+// The new dipole field of the tile's own sites goes to db_out; db_in (the
+// previous step's field, read for the advected dipole) is left untouched, so
+// neighbouring work-groups at other planes read consistent data.
+constexpr int FUSED3_NM = 8;  // per point: psi_r, psi_b, psi_s, rho_b - rho_r, rho_s, da_xyz
+
+template <int TX, int TY>
+struct Fused3Dims {
+    static constexpr int T = TX * TY;
+    static constexpr int MX = TX + 4, MY = TY + 4, MP = MX * MY;  // moments region
+    static constexpr int DX = TX + 2, DY = TY + 2, DP = DX * DY;  // dipole region
+};
+
+template <typename R, bool Dev, int TX, int TY, int BD>
+__device__ __forceinline__ void fused3_tile(const Phys<R> &P, const R *__restrict__ f,
+                                            const R *__restrict__ db_in, R *__restrict__ db_out,
+                                            R *__restrict__ fout, int x0, int y0, int z0, int zend,
+                                            R (&mring)[4][FUSED3_NM][Fused3Dims<TX, TY>::MP],
+                                            R (&dring)[3][3][Fused3Dims<TX, TY>::DP])
+{
+    using D = Fused3Dims<TX, TY>;
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool own = x < P.nx && y < P.ny;
+    auto mslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 2) & 3; };
+    auto dslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 1) % 3; };
+
+    // moments of plane zp at the tile + 2 sites
+    auto fillM = [&](int zp) __attribute__((always_inline)) {
+        const int sl = mslot(zp);
+        const int zw = wrap1(zp, P.nz);
+        for (int p = tid; p < D::MP; p += D::T) {
+            int gx = x0 - 2 + p % D::MX, gy = y0 - 2 + p / D::MX;
+            if (gx > P.nx + 1 || gy > P.ny + 1) continue;  // beyond a partial tile's needs
+            int xw = wrap1(gx, P.nx), yw = wrap1(gy, P.ny);
+            asm volatile("" : "+v"(xw), "+v"(yw));
+            R rho[3], da[3];
+            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xw, yw, zw, rho, da);
+            mring[sl][0][p] = psi_of<Dev>(rho[0], P.emr0[0]);
+            mring[sl][1][p] = psi_of<Dev>(rho[1], P.emr0[1]);
+            mring[sl][2][p] = psi_of<Dev>(rho[2], P.emr0[2]);
+            mring[sl][3][p] = rho[1] - rho[0];
+            mring[sl][4][p] = rho[2];
+            mring[sl][5][p] = da[0];
+            mring[sl][6][p] = da[1];
+            mring[sl][7][p] = da[2];
+        }
+    };
+    // dipole field of plane zp at the tile + 1 site; own sites of planes
+    // z0..zend-1 also go to db_out
+    auto fillD = [&](int zp) __attribute__((always_inline)) {
+        const int dsl = dslot(zp);
+        const int sl0 = mslot(zp), slm = mslot(zp - 1), slp = mslot(zp + 1);
+        for (int p = tid; p < D::DP; p += D::T) {
+            int dx = p % D::DX, dy = p / D::DX;
+            int gx = x0 - 1 + dx, gy = y0 - 1 + dy;
+            if (gx > P.nx || gy > P.ny) continue;
+            const int q = (dy + 1) * D::MX + (dx + 1);
+            R out[3];
+            dipole_core<R, Dev>(
+                P, mring[sl0][5][q], mring[sl0][6][q], mring[sl0][7][q], mring[sl0][4][q],
+                [&](int i) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sl0 : (CZ[i] > 0 ? slp : slm);
+                    return mring[sl][3][q + CY[i] * D::MX + CX[i]];
+                },
+                [&](int i, int a) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sl0 : (CZ[i] > 0 ? slp : slm);
+                    return mring[sl][5 + a][q + CY[i] * D::MX + CX[i]];
+                },
+                out);
+            dring[dsl][0][p] = out[0];
+            dring[dsl][1][p] = out[1];
+            dring[dsl][2][p] = out[2];
+            if (dx >= 1 && dx <= TX && dy >= 1 && dy <= TY && gx < P.nx && gy < P.ny && zp >= z0 && zp < zend) {
+                uint32_t sg = sidx(gx, gy, zp, P.nx, P.ny);
+                st_plane(db_out, sg, out[0]);
+                st_plane(db_out + (size_t)P.ns, sg, out[1]);
+                st_plane(db_out + (size_t)2 * P.ns, sg, out[2]);
+            }
+        }
+    };
+    // prologue: moments of planes z0-2 .. z0+1, dipoles of z0-1 and z0
+    fillM(z0 - 2);
+    fillM(z0 - 1);
+    fillM(z0);
+    fillM(z0 + 1);
+    __syncthreads();
+    fillD(z0 - 1);
+    fillD(z0);
+    __syncthreads();
+    const int qm = (ly + 2) * D::MX + (lx + 2), qd = (ly + 1) * D::DX + (lx + 1);
+    for (int k = z0; k < zend; k++) {
+        fillM(k + 2);
+        __syncthreads();
+        fillD(k + 1);
+        __syncthreads();
+        if (own) {
+            const int sm0 = mslot(k), smm = mslot(k - 1), smp = mslot(k + 1);
+            const int sd0 = dslot(k), sdm = dslot(k - 1), sdp = dslot(k + 1);
+            const R dbx[3] = {dring[sd0][0][qd], dring[sd0][1][qd], dring[sd0][2][qd]};
+            const R ps_own[3] = {mring[sm0][0][qm], mring[sm0][1][qm], mring[sm0][2][qm]};
+            R F[3][3];
+            sc_forces3<R, Dev>(
+                P, dbx, ps_own,
+                [&](int i, int c) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sm0 : (CZ[i] > 0 ? smp : smm);
+                    return mring[sl][c][qm + CY[i] * D::MX + CX[i]];
+                },
+                [&](int i, int a) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sd0 : (CZ[i] > 0 ? sdp : sdm);
+                    return dring[sl][a][qd + CY[i] * D::DX + CX[i]];
+                },
+                F);
+            bgk_site_g<R, Dev, 3, BD, true>(
+                P,
+                [&](int c, int i) __attribute__((always_inline)) {
+                    return pull<R, 3, false>(f, nullptr, P, c, i, x, y, k);
+                },
+                F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, bool Dev, int TX, int TY, int BD>
+__global__ __launch_bounds__(TX *TY) void k_fused3(Phys<R> P, const R *__restrict__ f,
+                                                    const R *__restrict__ db_in, R *__restrict__ db_out,
+                                                    R *__restrict__ fout, int lz, uint32_t xg)
+{
+    using D = Fused3Dims<TX, TY>;
+    __shared__ R mring[4][FUSED3_NM][D::MP];
+    __shared__ R dring[3][3][D::DP];
+    const uint32_t gx = ((uint32_t)P.nx + TX - 1) / TX, gy = ((uint32_t)P.ny + TY - 1) / TY;
+    const uint32_t gz = ((uint32_t)P.nz + lz - 1) / (uint32_t)lz;
+    const uint32_t tl = xcd_tile(blockIdx.x, gridDim.x, xg);
+    if (tl >= gx * gy * gz) return;  // uniform over the block
+    const int x0 = (int)(tl % gx) * TX, y0 = (int)((tl / gx) % gy) * TY, z0 = (int)(tl / (gx * gy)) * lz;
+    const int zend = min(z0 + lz, P.nz);
+    if (z0 >= zend) return;
+    fused3_tile<R, Dev, TX, TY, BD>(P, f, db_in, db_out, fout, x0, y0, z0, zend, mring, dring);
 }
 
 // --------------------------------------------------------------------------

@@ -328,8 +328,46 @@ class LauncherT : public Launcher {
         else
             k_fused2<R, Dev, false, TX, TY, SM><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz, xg);
     }
+    // LDS of k_fused3: the moments ring and the dipole ring
+    template <int TX, int TY>
+    static constexpr bool fits3()
+    {
+        using D3 = Fused3Dims<TX, TY>;
+        return (4 * FUSED3_NM * D3::MP + 9 * D3::DP) * sizeof(R) <= 160 * 1024;
+    }
+    template <int TX, int TY>
+    bool fused3_t(const DevBufs &b, int lz, hipStream_t s, bool launch)
+    {
+        if constexpr (!fits3<TX, TY>()) {
+            return false;
+        } else {
+            if (launch) {
+                unsigned grid =
+                    xcd_round(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY) * ((P.nz + lz - 1) / lz));
+                const R *f = (const R *)b.f[b.cur];
+                R *fo = (R *)b.f[1 - b.cur];
+                if (P.bdist == 0)
+                    k_fused3<R, Dev, TX, TY, 0><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.db, (R *)b.db_alt, fo,
+                                                                          lz, xg);
+                else
+                    k_fused3<R, Dev, TX, TY, 2><<<grid, TX * TY, 0, s>>>(P, f, (const R *)b.db, (R *)b.db_alt, fo,
+                                                                          lz, xg);
+            }
+            return true;
+        }
+    }
     bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, int st, hipStream_t s, bool launch)
     {
+        if (C == 3) {
+            // ternary: the reload form only, periodic lattices, nz >= 3 for the +-2 plane wrap
+            if (le || lz < 1 || st != 2 || P.nz < 3 || P.nx < 3 || P.ny < 3) return false;
+            if (tx == 32 && ty == 4) return fused3_t<32, 4>(b, lz, s, launch);
+            if (tx == 32 && ty == 8) return fused3_t<32, 8>(b, lz, s, launch);
+            if (tx == 64 && ty == 4) return fused3_t<64, 4>(b, lz, s, launch);
+            if (tx == 64 && ty == 8) return fused3_t<64, 8>(b, lz, s, launch);
+            if (tx == 32 && ty == 16) return fused3_t<32, 16>(b, lz, s, launch);
+            return false;
+        }
         if (C != 2 || lz < 1 || P.bdist != 2) return false;
         if (tx == 64 && ty == 4) return fused_t<64, 4>(b, lz, st, s, launch);
         if (tx == 32 && ty == 8) return fused_t<32, 8>(b, lz, st, s, launch);

@@ -16,6 +16,7 @@ namespace lb3d {
 struct DevBufs {
     void *f[2] = {nullptr, nullptr};
     void *aux = nullptr, *db = nullptr, *face_pc = nullptr, *face_aux = nullptr;
+    void *db_alt = nullptr;  // ternary one-pass plan: the new dipole field, swapped with db per step
     LEState *le = nullptr;
     int cur = 0;
 };

@@ -113,3 +113,67 @@ def test_autotune_keeps_state_and_picks_a_plan(ncomp):
     o.run(5)
     g.time_loop(5)
     assert_pops_close(o, g, what=f"after autotune ncomp={ncomp}")
+
+
+TERN_SHAPES = [(32, 4), (32, 8), (64, 4), (64, 8), (32, 16)]
+TERN_KW = dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003, amp=(0.02, 0.02, 0.01))
+
+
+def fused3_shapes(precision):
+    """Tile shapes of the ternary one-pass kernel (mode 3) that exist for this form."""
+    g = Lattice(8, 8, 8, ncomp=3, precision=precision)
+    ok = []
+    for tx, ty in TERN_SHAPES:
+        try:
+            g.set_plan(3, tx, ty, 4)
+            ok.append((tx, ty))
+        except LB3DError:
+            pass
+    return ok
+
+
+def test_ternary_one_pass_shapes_exist():
+    assert len(fused3_shapes(LB3D_FP32_DEV)) == 5
+    assert (32, 4) in fused3_shapes(LB3D_FP64)
+    g = Lattice(8, 8, 8, ncomp=3)
+    with pytest.raises(LB3DError):
+        g.set_plan(2, 32, 4, 4)  # the register and LDS stages are binary only
+
+
+@pytest.mark.parametrize("precision", [LB3D_FP64, LB3D_FP32_DEV])
+@pytest.mark.parametrize("dims,bdist", [((20, 13, 11), 2), ((7, 5, 3), 2), ((33, 9, 6), 0)])
+def test_ternary_one_pass_equals_three_pass_bitwise(precision, dims, bdist):
+    """Populations and the dipole field after 7 steps, for every tile shape,
+    z-chunks of 1, 3 and nz, lattices that do not divide into tiles."""
+    kw = dict(TERN_KW, precision=precision, bdist=bdist)
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0)
+    ref.time_loop(7)
+    want = ref.populations()
+    want_db = ref.download_sites()[1:-1, 1:-1, 1:-1, 75:78]
+    for tx, ty in fused3_shapes(precision):
+        for lz in (1, 3, dims[2]):
+            _, g = pair(*dims, **kw)
+            g.set_plan(3, tx, ty, lz, xcd_group=0)
+            g.time_loop(7)
+            got = g.populations()
+            assert np.array_equal(got, want), \
+                f"{dims} bdist {bdist} {tx}x{ty} lz={lz}: max {np.abs(got - want).max()}"
+            assert np.array_equal(g.download_sites()[1:-1, 1:-1, 1:-1, 75:78], want_db)
+
+
+def test_ternary_one_pass_fp64_matches_oracle_and_downloads():
+    """24x16x12 ternary, 20 steps against the oracle; the staged surface and
+    the downloaded moments (rho, da of the last advection) as the oracle's."""
+    o, g = pair(24, 16, 12, **TERN_KW)
+    g.set_plan(3, 32, 8, 4)
+    o.run(17)
+    g.time_loop(17)
+    for _ in range(3):
+        g.time_step()
+    o.run(3)
+    assert_pops_close(o, g, what="ternary one-pass 24x16x12 x20")
+    N = g.download_sites()
+    I = o.interior()
+    for off in (19, 43, 67, 72, 73, 74, 75, 76, 77):
+        np.testing.assert_allclose(N[1:-1, 1:-1, 1:-1, off], I[..., off], rtol=0, atol=1e-13, err_msg=str(off))
