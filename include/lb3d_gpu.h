@@ -72,6 +72,11 @@ typedef struct lb3d_params {
     int32_t g_accn_min[3];    /* body-force box, global 1-based inclusive (x, y, z); */
     int32_t g_accn_max[3];    /*   max 0 means the lattice extent (code/lb3d_config.F90:931-933) */
     double rho0[3];           /* FP32_DEV reference densities (0 picks 0.7) */
+    /* walls (rock sites come with the uploaded sites' rock_state; binary only):
+     * wall couplings and relaxation weights (code/lb3d_config.F90:260-261) and
+     * the colours rock sites carry (lbe_init_rock_colour, code/lb3d_ic_rock.F90:146-189) */
+    double g_wr, g_wb, tau_wr, tau_wb;
+    double rock_colour_r, rock_colour_b;
 } lb3d_params;
 
 typedef struct lb3d_ctx lb3d_ctx;

@@ -65,6 +65,7 @@ struct lb3d_ctx {
     LEState host_le{};
     hipGraphExec_t graph = nullptr;
     double *d_partials = nullptr;
+    uint8_t *rock_dev = nullptr;  // rock mask (lattices with walls)
     size_t f_bytes = 0, aux_bytes = 0, db_bytes = 0;
     int naux = 0;
     Plan plan;
@@ -143,7 +144,8 @@ void clear_error() { g_err.clear(); }
 double rho0_of(const lb3d_params &p, int k) { return p.rho0[k] > 0.0 ? p.rho0[k] : 0.7; }
 
 int aos_to_device_layout(const lb3d_params &p, const double *N, int h, int sd,
-                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes)
+                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes,
+                         std::vector<uint8_t> *rock, size_t *nrock)
 {
     const int C = p.ncomp;
     if (sd != (C == 3 ? 79 : 49)) return fail(LB3D_EINVAL, "site_doubles must be 79 (ternary) or 49 (binary)");
@@ -160,12 +162,18 @@ int aos_to_device_layout(const lb3d_params &p, const double *N, int h, int sd,
     double *fd = (double *)f_bytes.data();
     float *dbf = (float *)db_bytes.data();
     double *dbd = (double *)db_bytes.data();
+    size_t nr = 0;
+    if (rock) rock->assign(ns, 0);
     for (int z = 0; z < nz; z++)
         for (int y = 0; y < ny; y++)
             for (int x = 0; x < nx; x++) {
                 const double *S = N + ((size_t)(x + h) + lx * ((size_t)(y + h) + ly * (size_t)(z + h))) * sd;
                 size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
-                if (S[rock_off] != 0.0) return fail(LB3D_EINVAL, "rock sites are not supported (rock_state must be 0)");
+                if (S[rock_off] != 0.0) {
+                    if (!rock) return fail(LB3D_EINVAL, "rock sites are not supported here (rock_state must be 0)");
+                    (*rock)[s] = 1;
+                    nr++;
+                }
                 for (int k = 0; k < C; k++)
                     for (int i = 0; i < 19; i++) {
                         double v = S[24 * k + i];
@@ -183,6 +191,7 @@ int aos_to_device_layout(const lb3d_params &p, const double *N, int h, int sd,
                             dbd[k * ns + s] = S[75 + k];
                     }
             }
+    if (nrock) *nrock = nr;
     return LB3D_OK;
 }
 
@@ -283,6 +292,7 @@ int lb3d_gpu_destroy(lb3d_ctx *c)
     if (c->b.face_aux) (void)hipFree(c->b.face_aux);
     if (c->b.le) (void)hipFree(c->b.le);
     if (c->d_partials) (void)hipFree(c->d_partials);
+    if (c->rock_dev) (void)hipFree(c->rock_dev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c->L;
     delete c;
@@ -307,13 +317,44 @@ static bool is_dev(const lb3d_ctx *c) { return c->p.precision == LB3D_FP32_DEV; 
 static double rho0_of(const lb3d_ctx *c, int k) { return lb3d::rho0_of(c->p, k); }
 static double w_of(int i) { return i < 18 ? 1.0 / 36.0 : 1.0 / 3.0; }
 
+// Rock mask on the device (nullptr: no walls).  With walls only the two-pass
+// plan exists, so a one-pass plan falls back to it.
+static int set_walls(lb3d_ctx *c, const uint8_t *mask)
+{
+    const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    if (!mask) {
+        c->L->walls = false;
+        c->b.rock = nullptr;
+        return LB3D_OK;
+    }
+    if (!c->rock_dev) HIP_TRY(hipMalloc((void **)&c->rock_dev, ns));
+    HIP_TRY(hipMemcpyAsync(c->rock_dev, mask, ns, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->b.rock = c->rock_dev;
+    c->L->walls = true;
+    if (c->plan.fused) {
+        c->plan = Plan();
+        c->L->xg = c->plan.xg;
+    }
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return LB3D_OK;
+}
+
 int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, int32_t nt0)
 {
     g_err.clear();
     if (!c || !N) return fail(LB3D_EINVAL, "NULL argument");
     std::vector<unsigned char> fb, dbb;
-    int rc = aos_to_device_layout(c->p, N, h, sd, fb, dbb);
+    std::vector<uint8_t> rock;
+    size_t nrock = 0;
+    int rc = aos_to_device_layout(c->p, N, h, sd, fb, dbb, &rock, &nrock);
     if (rc) return rc;
+    if (nrock && (c->p.ncomp != 2 || c->le))
+        return fail(LB3D_EINVAL, "rock sites (walls) are supported on periodic binary lattices");
+    if ((rc = set_walls(c, nrock ? rock.data() : nullptr))) return rc;
     c->b.cur = 0;
     HIP_TRY(hipMemcpyAsync(c->b.f[0], fb.data(), c->f_bytes, hipMemcpyHostToDevice, c->stream));
     if (c->p.ncomp == 3)
@@ -332,6 +373,8 @@ int lb3d_gpu_init_noise(lb3d_ctx *c, uint64_t seed, const double rho_mean[3], co
 {
     g_err.clear();
     if (!c || !rho_mean || !amp) return fail(LB3D_EINVAL, "NULL argument");
+    int rc = set_walls(c, nullptr);
+    if (rc) return rc;
     c->b.cur = 0;
     c->L->init_noise(c->b, seed, rho_mean, amp, dipoles, c->stream);
     HIP_TRY(hipGetLastError());
@@ -397,6 +440,12 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
     }
     if (c->aux_valid && (rc = fetch(c->b.aux, (size_t)c->naux * ns, aux))) return rc;
     if (C == 3 && (rc = fetch(c->b.db, 3 * ns, db))) return rc;
+    std::vector<uint8_t> rock;
+    if (c->b.rock) {
+        rock.resize(ns);
+        HIP_TRY(hipMemcpyAsync(rock.data(), c->b.rock, ns, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     const size_t lx = nx + 2 * h, ly = ny + 2 * h, lz = nz + 2 * h;
     const int rock_off = C == 3 ? 78 : 48;
     auto rec = [&](size_t x, size_t y, size_t z) { return N + (x + lx * (y + ly * z)) * sd; };
@@ -430,6 +479,12 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
                         S[75 + a] = db[(size_t)a * ns + s];
                     }
                 S[rock_off] = 0.0;
+                if (!rock.empty() && rock[s]) {
+                    // rock sites with the colours lbe_init_rock_colour gives them
+                    S[rock_off] = 1.0;
+                    if (c->p.rock_colour_r > 0) S[23] = c->p.rock_colour_r;
+                    if (c->p.rock_colour_b > 0) S[24 + 23] = c->p.rock_colour_b;
+                }
             }
     // halo: periodic images of the interior (x, then y over full x, then z)
     for (size_t z = h; z < (size_t)nz + h; z++)

@@ -97,6 +97,22 @@ __device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__
     return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), sidx(xs, ys, zs, P.nx, P.ny));
 }
 
+// Pull with walls (periodic binary lattices with rock sites): a rock site
+// keeps its own populations (code/lb3d_advection.F90:126), and a fluid site
+// whose source x - c_i is rock takes its own n_bounce(i) instead, the
+// half-way bounce-back of code/lb3d_bc_bounce.F90:53-90.
+template <typename R, int C>
+__device__ __forceinline__ R pull_rock(const R *__restrict__ f, const uint8_t *__restrict__ rock,
+                                       const Phys<R> &P, int c, int i, int x, int y, int z, bool self_rock)
+{
+    const uint32_t s = sidx(x, y, z, P.nx, P.ny);
+    if (self_rock) return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), s);
+    int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny), zs = wrap1(z - CZ[i], P.nz);
+    uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
+    if (rock[q]) return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)BOUNCE[i], P.ns), s);
+    return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), q);
+}
+
 // Value k of the aux field at the neighbour (x+dx, y+dy, z+dz); for LE the
 // x-halo neighbours read face_aux.
 template <typename R, bool LE>
@@ -134,20 +150,27 @@ __device__ __forceinline__ R psi_of(R rho, R emr0)
 // (code/lb3d_analysis.F90:64-76); ternary also the advected dipole
 // da = sum_i g_i n_s,i(x-c_i) db(x-c_i) (:130-148).  The streamed
 // populations themselves are not stored: k_collide pulls them again.
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false>
 __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, int x, int y, int z, R (&rho)[C],
-                                             R (&da)[3])
+                                             R (&da)[3], const uint8_t *__restrict__ rock = nullptr)
 {
+    static_assert(!Rock || (C == 2 && !LE), "walls: periodic binary lattices");
 #pragma unroll
     for (int c = 0; c < C; c++) rho[c] = R(0);
     R da0 = R(0), da1 = R(0), da2 = R(0);
+    bool self_rock = false;
+    if constexpr (Rock) self_rock = rock[sidx(x, y, z, P.nx, P.ny)] != 0;
 #pragma unroll
     for (int i = 0; i < 19; i++) {
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            R v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+            R v;
+            if constexpr (Rock)
+                v = pull_rock<R, C>(f, rock, P, c, i, x, y, z, self_rock);
+            else
+                v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
             rho[c] = rho[c] + v * R(GD[i]);
             if constexpr (C == 3) {
                 if (c == 2) {
@@ -168,16 +191,16 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
     da[2] = da2;
 }
 
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false>
 __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, R *__restrict__ aux,
-                                             uint32_t s)
+                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr)
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R rho[C], da[3];
-    moments_core<R, Dev, C, LE>(P, f, face_pc, db, x, y, z, rho, da);
+    moments_core<R, Dev, C, LE, Rock>(P, f, face_pc, db, x, y, z, rho, da, rock);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
@@ -190,15 +213,15 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
     }
 }
 
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false>
 __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ db, R *__restrict__ aux,
-                                                 uint32_t xg)
+                                                 uint32_t xg, const uint8_t *__restrict__ rock)
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    moments_site<R, Dev, C, LE>(P, f, face_pc, db, aux, s);
+    moments_site<R, Dev, C, LE, Rock>(P, f, face_pc, db, aux, s, rock);
 }
 
 // --------------------------------------------------------------------------
@@ -344,6 +367,54 @@ __device__ __forceinline__ void sc_forces3(const Phys<R> &P, const R (&dbx)[3], 
 // --------------------------------------------------------------------------
 // Shan-Chen forces at site (x,y,z) from the aux field.
 // binary: code/lb3d_force_shanchen.F90:127-210; ternary: :215-330.
+// Binary Shan-Chen forces next to walls (code/lb3d_force_shanchen.F90:127-210):
+// a rock neighbour enters the wall sums with psi(rho + rock colour) and the
+// wall couplings g_wr, g_wb weighted by tau_w/tau; fluid neighbours as usual.
+// psi values are used whole (FP32 form: psi0 + deviation), since the wall
+// and fluid sums no longer run over all 18 directions.
+template <typename R, bool Dev>
+__device__ __forceinline__ void sc_forces_wall(const Phys<R> &P, const R *__restrict__ aux,
+                                               const uint8_t *__restrict__ rock, int x, int y, int z,
+                                               uint32_t s, R F[2][3])
+{
+    R frb[3] = {R(0), R(0), R(0)}, fbr[3] = {R(0), R(0), R(0)};
+    R fwr[3] = {R(0), R(0), R(0)}, fwb[3] = {R(0), R(0), R(0)};
+#pragma unroll
+    for (int i = 0; i < 18; i++) {
+        int xa = wrap1(x + CX[i], P.nx), ya = wrap1(y + CY[i], P.ny), za = wrap1(z + CZ[i], P.nz);
+        uint32_t q = sidx(xa, ya, za, P.nx, P.ny);
+        R pr, pb;
+        const bool wall = rock[q] != 0;
+        if (wall) {
+            pr = psi_of<Dev>(ld_plane(plane_of(aux, 2u, P.ns), q) + P.rock_col[0], P.emr0[0]);
+            pb = psi_of<Dev>(ld_plane(plane_of(aux, 3u, P.ns), q) + P.rock_col[1], P.emr0[1]);
+        } else {
+            pr = ld_plane(aux, q);
+            pb = ld_plane(plane_of(aux, 1u, P.ns), q);
+        }
+        if constexpr (Dev) {
+            pr = P.psi0[0] + pr;
+            pb = P.psi0[1] + pb;
+        }
+        R green_r = pb * R(GD[i]), green_b = pr * R(GD[i]);
+        R *sr = wall ? fwb : frb, *sb = wall ? fwr : fbr;
+        if (CX[i] != 0) { sr[0] = sr[0] + green_r * R(CX[i]); sb[0] = sb[0] + green_b * R(CX[i]); }
+        if (CY[i] != 0) { sr[1] = sr[1] + green_r * R(CY[i]); sb[1] = sb[1] + green_b * R(CY[i]); }
+        if (CZ[i] != 0) { sr[2] = sr[2] + green_r * R(CZ[i]); sb[2] = sb[2] + green_b * R(CZ[i]); }
+    }
+    R psr = ld_plane(aux, s), psb = ld_plane(plane_of(aux, 1u, P.ns), s);
+    if constexpr (Dev) {
+        psr = P.psi0[0] + psr;
+        psb = P.psi0[1] + psb;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        R wb = -(P.g_wb * fwb[k]), wr = -(P.g_wr * fwr[k]);
+        F[1][k] = ((-(P.g_br * fbr[k])) + (wb / P.tau[1]) * P.tau_wb) * psb;
+        F[0][k] = ((-(P.g_br * frb[k])) + (wr / P.tau[0]) * P.tau_wr) * psr;
+    }
+}
+
 template <typename R, bool Dev, int C, bool LE>
 __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict__ aux,
                                           const R *__restrict__ face_aux,
@@ -490,37 +561,53 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
 // --------------------------------------------------------------------------
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
-template <typename R, bool Dev, int C, bool LE, int BD>
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false>
 __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ aux,
                                              const R *__restrict__ face_aux,
                                              const R *__restrict__ db, R *__restrict__ fout,
-                                             uint32_t s)
+                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr)
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R n[C][19];
-#pragma unroll
-    for (int i = 0; i < 19; i++)
-#pragma unroll
-        for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
     R F[C][3];
-    sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+    if constexpr (Rock) {
+        static_assert(C == 2 && !LE, "walls: periodic binary lattices");
+        const bool self_rock = rock[s] != 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+#pragma unroll
+            for (int c = 0; c < C; c++) n[c][i] = pull_rock<R, C>(f, rock, P, c, i, x, y, z, self_rock);
+        // only fluid sites feel the Shan-Chen force (code/lb3d_collision.F90:160-172)
+        if (self_rock) {
+#pragma unroll
+            for (int c = 0; c < C; c++) F[c][0] = F[c][1] = F[c][2] = R(0);
+        } else {
+            sc_forces_wall<R, Dev>(P, aux, rock, x, y, z, s, F);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+#pragma unroll
+            for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+        sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+    }
     bgk_site<R, Dev, C, BD>(P, n, F, x, y, z, s, fout);
 }
 
-template <typename R, bool Dev, int C, bool LE, int BD>
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false>
 __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ aux,
                                                  const R *__restrict__ face_aux,
                                                  const R *__restrict__ db, R *__restrict__ fout,
-                                                 uint32_t xg)
+                                                 uint32_t xg, const uint8_t *__restrict__ rock)
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    collide_site<R, Dev, C, LE, BD>(P, f, face_pc, aux, face_aux, db, fout, s);
+    collide_site<R, Dev, C, LE, BD, Rock>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
 }
 
 // --------------------------------------------------------------------------

@@ -14,6 +14,8 @@ __device__ constexpr int CX[19] = {1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1
 __device__ constexpr int CY[19] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0};
 __device__ constexpr int CZ[19] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0};
 __device__ constexpr int GD[19] = {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+// opposite vector (code/lb3d_lattice.F90:173-177, 0-based)
+__device__ constexpr int BOUNCE[19] = {1, 0, 3, 2, 5, 4, 11, 10, 13, 12, 7, 6, 9, 8, 17, 16, 15, 14, 18};
 
 // Host copies of the same tables.
 namespace host {
@@ -42,6 +44,8 @@ struct Phys {
     R g_accn[3];
     // deviation storage: w_i*rho0 per component and psi(rho0), exp(-rho0)
     R rho0[3], wrho0_move[3], wrho0_rest[3], psi0[3], emr0[3];
+    // walls: g_wr, g_wb, tau_wr/tau_r, tau_wb/tau_b as the force uses them, rock colours
+    R g_wr, g_wb, tau_wr, tau_wb, rock_col[2];
 };
 
 // Lees-Edwards scalars, updated on the device once per step (FP64).

@@ -50,6 +50,12 @@ Phys<R> make_phys(const lb3d_params &p)
     P.g_ss = R(p.g_ss);
     P.two_g_bs = R(2.0 * p.g_bs);
     P.sign_g_bs = R(std::signbit(p.g_bs) ? -1.0 : 1.0);
+    P.g_wr = R(p.g_wr);
+    P.g_wb = R(p.g_wb);
+    P.tau_wr = R(p.tau_wr);
+    P.tau_wb = R(p.tau_wb);
+    P.rock_col[0] = R(p.rock_colour_r);
+    P.rock_col[1] = R(p.rock_colour_b);
     return P;
 }
 
@@ -105,13 +111,14 @@ __global__ __launch_bounds__(256) void k_init_noise(Phys<R> P, uint64_t seed, do
 // populations: per-block FP64 partial sums, reduced on the host.
 template <typename R, bool Dev, int C>
 __global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restrict__ f,
+                                                     const uint8_t *__restrict__ rock,
                                                      double *__restrict__ partials)
 {
     __shared__ double red[NDIAG][256];
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NDIAG];
     for (int k = 0; k < NDIAG; k++) v[k] = 0.0;
-    if (s < P.ns) {
+    if (s < P.ns && !(rock && rock[s])) {  // fluid sites only
         bool isnan_ = false, isneg = false, fast = false;
         for (int c = 0; c < C; c++) {
             double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
@@ -161,13 +168,14 @@ __global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restr
 // Partials are reduced on the host.
 template <typename R, bool Dev, int C>
 __global__ __launch_bounds__(256) void k_sanity(Phys<R> P, const R *__restrict__ f,
+                                                const uint8_t *__restrict__ rock,
                                                 double *__restrict__ partials)
 {
     __shared__ double red[NSAN][256];
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NSAN];
     for (int k = 0; k < NSAN; k++) v[k] = k < SAN_MAX ? 0.0 : -1.0;
-    if (s < P.ns) {
+    if (s < P.ns && !(rock && rock[s])) {  // fluid sites only (no_rock, code/lb3d_sanity.F90:134)
         int x, y, z;
         site_xyz(s, P.nx, P.ny, x, y, z);
         double key = ((double)x * (double)P.ny + (double)y) * (double)P.nz + (double)z;
@@ -245,11 +253,16 @@ class LauncherT : public Launcher {
     {
         const R *f = (const R *)b.f[b.cur];
         if (C == 3)
-            k_moments<R, Dev, 3, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux, xg);
+            k_moments<R, Dev, 3, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux, xg,
+                                                                  nullptr);
         else if (le)
-            k_moments<R, Dev, 2, true><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux, xg);
+            k_moments<R, Dev, 2, true><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux, xg,
+                                                                 nullptr);
+        else if (b.rock)
+            k_moments<R, Dev, 2, false, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg,
+                                                                        b.rock);
         else
-            k_moments<R, Dev, 2, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg);
+            k_moments<R, Dev, 2, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg, nullptr);
     }
     void dipole(const DevBufs &b, hipStream_t s) override
     {
@@ -262,14 +275,17 @@ class LauncherT : public Launcher {
         R *fo = (R *)b.f[1 - b.cur];
         if (C == 3)
             k_collide<R, Dev, 3, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      (const R *)b.db, fo, xg);
+                                                                      (const R *)b.db, fo, xg, nullptr);
         else if (le)
             k_collide<R, Dev, 2, true, BD><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc,
                                                                      (const R *)b.aux, (const R *)b.face_aux,
-                                                                     nullptr, fo, xg);
+                                                                     nullptr, fo, xg, nullptr);
+        else if (b.rock)
+            k_collide<R, Dev, 2, false, BD, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
+                                                                            nullptr, nullptr, fo, xg, b.rock);
         else
             k_collide<R, Dev, 2, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      nullptr, fo, xg);
+                                                                      nullptr, fo, xg, nullptr);
     }
     void collide(const DevBufs &b, hipStream_t s) override
     {
@@ -358,6 +374,7 @@ class LauncherT : public Launcher {
     }
     bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, int st, hipStream_t s, bool launch)
     {
+        if (walls) return false;  // walls run on the two-pass kernels
         if (C == 3) {
             // ternary: the reload form only, periodic lattices, nz >= 3 for the +-2 plane wrap
             if (le || lz < 1 || st != 2 || P.nz < 3 || P.nx < 3 || P.ny < 3) return false;
@@ -402,17 +419,17 @@ class LauncherT : public Launcher {
     {
         const R *f = (const R *)b.f[b.cur];
         if (C == 3)
-            k_sanity<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+            k_sanity<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, b.rock, partials);
         else
-            k_sanity<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+            k_sanity<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, b.rock, partials);
     }
     void diagnostics(const DevBufs &b, double *partials, hipStream_t s) override
     {
         const R *f = (const R *)b.f[b.cur];
         if (C == 3)
-            k_diagnostics<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+            k_diagnostics<R, Dev, 3><<<blocks(), BLOCK, 0, s>>>(P, f, b.rock, partials);
         else
-            k_diagnostics<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, partials);
+            k_diagnostics<R, Dev, 2><<<blocks(), BLOCK, 0, s>>>(P, f, b.rock, partials);
     }
 
   private:

@@ -17,6 +17,7 @@ struct DevBufs {
     void *f[2] = {nullptr, nullptr};
     void *aux = nullptr, *db = nullptr, *face_pc = nullptr, *face_aux = nullptr;
     void *db_alt = nullptr;  // ternary one-pass plan: the new dipole field, swapped with db per step
+    const uint8_t *rock = nullptr;  // 1 at rock sites, or nullptr for a lattice without walls
     LEState *le = nullptr;
     int cur = 0;
 };
@@ -28,6 +29,7 @@ class Launcher {
     // chunk of consecutive tiles per XCD in the stepping kernels (xcd_tile);
     // 1 is the plain round-robin order
     uint32_t xg = 1;
+    bool walls = false;  // the lattice has rock sites (the one-pass plans are then unavailable)
     virtual void moments(const DevBufs &b, hipStream_t s) = 0;
     virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
     virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
@@ -90,7 +92,8 @@ void clear_error();
 // device layout of one lattice: populations (FP64, or FP32 deviations from
 // w_i*rho0) and the ternary dipole field.  Returns 0 or an LB3D_E* code.
 int aos_to_device_layout(const lb3d_params &p, const double *whole_N, int halo, int site_doubles,
-                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes);
+                         std::vector<unsigned char> &f_bytes, std::vector<unsigned char> &db_bytes,
+                         std::vector<uint8_t> *rock = nullptr, size_t *nrock = nullptr);
 // The inverse for the populations: device bytes -> FP64 populations.
 void device_layout_to_pops(const lb3d_params &p, const unsigned char *f_bytes, double *out);
 double rho0_of(const lb3d_params &p, int c);

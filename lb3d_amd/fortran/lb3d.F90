@@ -58,6 +58,7 @@ contains
     integer :: c, s
     integer, parameter :: g(19) = [2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1]
     real(rk) :: rsum
+    logical :: walls
     character(len=1024) :: rep
     integer :: k
     call log_msg('lb3d on MI355X: reading '//trim(inp))
@@ -78,6 +79,8 @@ contains
        read(restore_string(2:9), *) tstart
     end if
     nt = tstart
+    ! lb3d_ic_init stage 4: the initial condition, then the rock (code/lb3d_ic.F90:76-97)
+    walls = lb3d_init_rock()
 
     p%nx = nx; p%ny = ny; p%nz = nz
     p%ncomp = ncomp
@@ -94,6 +97,8 @@ contains
     p%g_accn = [g_accn_x, g_accn_y, g_accn]
     p%g_accn_min = [g_accn_min_x, g_accn_min_y, g_accn_min]
     p%g_accn_max = [g_accn_max_x, g_accn_max_y, g_accn_max]
+    p%g_wr = g_wr; p%g_wb = g_wb; p%tau_wr = tau_wr; p%tau_wb = tau_wb
+    p%rock_colour_r = rock_colour_r; p%rock_colour_b = rock_colour_b
     ! reference densities of the FP32 deviation form: the mean of the initial state
     p%rho0 = 0.0_c_double
     if (restored) then
@@ -117,6 +122,13 @@ contains
     else
        call check(lb3d_gpu_init_noise(ctx, int(seed, c_int64_t), [fr, fb, fg], [pr, pb, pg], 1_c_int32_t), &
             'init_noise')
+       if (walls) then
+          ! the rock goes onto the generated state
+          call check(lb3d_gpu_download_sites(ctx, whole_N, 1_c_int32_t, int(sd, c_int32_t)), 'download_sites')
+          walls = lb3d_init_rock()
+          call check(lb3d_gpu_upload_sites(ctx, whole_N, 1_c_int32_t, int(sd, c_int32_t), &
+               int(tstart, c_int32_t)), 'upload_sites')
+       end if
     end if
     call lb3d_bc_init(4)
     if (autotune) then

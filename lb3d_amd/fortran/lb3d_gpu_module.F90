@@ -31,6 +31,8 @@ module lb3d_gpu_module
      integer(c_int32_t) :: g_accn_min(3)
      integer(c_int32_t) :: g_accn_max(3)
      real(c_double) :: rho0(3)
+     real(c_double) :: g_wr, g_wb, tau_wr, tau_wb
+     real(c_double) :: rock_colour_r, rock_colour_b
   end type lb3d_params
 
   !> lb3d_sanity: C [species][axis][xyz] arrays appear here as (xyz, axis, species)

@@ -32,19 +32,21 @@ module lb3d_host_config
   character(len=256) :: folder = 'output', cpfolder = '.', srccpfolder = '', gr_out_file = 'lb3d'
   real(rk) :: fr = 0.7_rk, fg = 0.7_rk, fb = 0.7_rk, fd = 0.0_rk, fr1 = 0.0_rk, fr2 = 0.0_rk
   real(rk) :: pr = 0.01_rk, pg = 0.0_rk, pb = 0.01_rk, rock_colour = 0.0_rk, beta = 1.0_rk
+  real(rk) :: rock_colour_r = 0.0_rk, rock_colour_b = 0.0_rk
   real(rk) :: drop_xshift = 0.0_rk, drop_yshift = 0.0_rk, drop_zshift = 0.0_rk
   namelist /variable_input/ n_iteration, n_sci_start, sci_int, n_sci_int, sci_sur, n_sci_sur, &
        sci_od, n_sci_od, sci_wd, n_sci_wd, sci_dir, n_sci_dir, sci_vel, n_sci_vel, sci_flo, &
        n_sci_flo, sci_arrows, n_sci_arrows, sci_pressure, n_sci_pressure, sci_pressure_init, &
        sci_velocities, n_sci_velocities, sci_rock, n_sci_rock, sci_fluxz, n_sci_fluxz, &
        sci_profile, n_sci_profile, post, folder, cpfolder, srccpfolder, gr_out_file, init_cond, &
-       fr, fg, fb, fd, fr1, fr2, pr, pb, pg, inv_fluid, inv_type, rock_colour, beta, &
-       drop_xshift, drop_yshift, drop_zshift
+       fr, fg, fb, fd, fr1, fr2, pr, pb, pg, inv_fluid, inv_type, rock_colour, rock_colour_r, &
+       rock_colour_b, beta, drop_xshift, drop_yshift, drop_zshift
 
   ! &lbe_input (defaults of code/lb3d_config.F90:244-297)
   real(rk) :: amass_r = 1.0_rk, amass_b = 1.0_rk, amass_s = 1.0_rk
   real(rk) :: tau_r = 1.0_rk, tau_b = 1.0_rk, tau_s = 1.0_rk, tau_d = 2.0_rk
   real(rk) :: g_br = 0.0_rk, g_bs = 0.0_rk, g_ss = 0.0_rk, g_wr = 0.0_rk, d_0 = 1.0_rk
+  real(rk) :: g_wb = 0.0_rk, tau_wr = 1.0_rk, tau_wb = 1.0_rk
   real(rk) :: perturbation = 0.0_rk, shear_u = 0.0_rk, shear_omega = 0.0_rk, acccoef = 0.0_rk
   real(rk) :: g_accn = 0.0_rk, g_accn_x = 0.0_rk, g_accn_y = 0.0_rk
   integer :: g_accn_min = 0, g_accn_max = 0, g_accn_min_x = 0, g_accn_max_x = 0
@@ -56,7 +58,8 @@ module lb3d_host_config
   character(len=64) :: restore_string = 't00000000-0000000000', dump_format = 'xdr'
   character(len=64) :: checkpoint_format = 'xdr'
   namelist /lbe_input/ amass_b, amass_r, amass_s, tau_b, tau_r, tau_s, tau_d, g_br, g_bs, g_ss, &
-       g_wr, d_0, bdist, perturbation, shear_u, shear_omega, g_accn, g_accn_x, g_accn_y, &
+       g_wr, g_wb, tau_wr, tau_wb, d_0, bdist, perturbation, shear_u, shear_omega, g_accn, &
+       g_accn_x, g_accn_y, &
        g_accn_min, g_accn_max, g_accn_min_x, g_accn_max_x, g_accn_min_y, g_accn_max_y, &
        n_checkpoint, restore_string, num_chkp_files, psifunc, dump_format, checkpoint_safe, &
        checkpoint_format, write_AVS_fld, dump_double, n_sanity_check, SCMP, OXFORD, bcsel, acccoef
@@ -247,6 +250,69 @@ contains
     restored = .true.
   end subroutine read_checkpoint_xdr
 
+  !> lbe_init_rock (code/lb3d_ic_rock.F90:50-189, one rank, no XDRROCKWET):
+  !> read obs_folder/obs_file (one big-endian float32 per site, z outer ..
+  !> x inner, read_rock_xdrf_par, code/lb3d_io_xdrf.F90:400-470): a site is
+  !> rock where the value exceeds 0.  Rock sites lose their populations (and
+  !> ternary da, db) and take the positive rock colours.  Returns whether any
+  !> rock was set.
+  logical function lb3d_init_rock() result(any_rock)
+    character(len=512) :: fname
+    integer :: u, ios, x, y, z, c
+    integer(1), allocatable :: bytes(:)
+    real(4), allocatable :: vals(:)
+    integer :: k
+    any_rock = .false.
+    ! rock_colour is the deprecated form of rock_colour_r / rock_colour_b
+    ! (code/lb3d_config.F90:657-665)
+    if (rock_colour > 0) then
+       rock_colour_r = rock_colour
+    else if (rock_colour < 0) then
+       rock_colour_b = rock_colour
+    end if
+    if (trim(obs_file) == 'empty.dat') return
+    fname = trim(obs_folder)//'/'//trim(obs_file)
+    if (index(fname, '.xdr') == 0) then
+       write(*, '(A,A)') 'FATAL ERROR: only XDR obstacle files are read: ', trim(fname)
+       error stop 1
+    end if
+    call log_msg('Reading rock data file <'//trim(fname)//'>')
+    open(newunit=u, file=fname, access='stream', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       write(*, '(A,A)') 'FATAL ERROR: cannot open obstacle file ', trim(fname)
+       error stop 1
+    end if
+    allocate(bytes(4 * nx * ny * nz), vals(nx * ny * nz))
+    read(u, iostat=ios) bytes
+    close(u)
+    if (ios /= 0) then
+       write(*, '(A)') 'FATAL ERROR: short obstacle file'
+       error stop 1
+    end if
+    call swap4(bytes)
+    vals = transfer(bytes, vals)
+    k = 0
+    do z = 1, nz
+       do y = 1, ny
+          do x = 1, nx
+             k = k + 1
+             if (vals(k) > 0.0) then
+                whole_N(sd, x, y, z) = 1.0_rk
+                do c = 1, ncomp
+                   whole_N(24 * (c - 1) + 1:24 * (c - 1) + 19, x, y, z) = 0.0_rk
+                end do
+                if (ncomp == 3) whole_N(73:78, x, y, z) = 0.0_rk
+                if (rock_colour_r > 0) whole_N(24, x, y, z) = rock_colour_r
+                if (rock_colour_b > 0) whole_N(48, x, y, z) = rock_colour_b
+                any_rock = .true.
+             else
+                whole_N(sd, x, y, z) = 0.0_rk
+             end if
+          end do
+       end do
+    end do
+  end function lb3d_init_rock
+
   !> Write whole_N in the reference's XDR checkpoint layout (one rank).
   subroutine write_checkpoint_xdr()
     character(len=512) :: fname
@@ -362,6 +428,8 @@ contains
                    rho = rho + whole_N(24 * (c - 1) + s, x, y, z) * g(s)
                 end do
                 sc(x, y, z) = amass(c) * rho
+                ! od and wd are 0 on rock sites (dump_od/dump_wd, code/lb3d_io_dump_data.F90:1120, 1156)
+                if (c < 3 .and. whole_N(sd, x, y, z) /= 0.0_rk) sc(x, y, z) = 0.0_rk
              end do
           end do
        end do
@@ -391,6 +459,10 @@ contains
                    den = den + rho * amass(c) * omega(c)
                 end do
                 v = v / max(10.d-9, den)
+                if (whole_N(sd, x, y, z) /= 0.0_rk) then   ! rock: 0 (dump_vel, :1031, 1086)
+                   sc(x, y, z) = 0.0_rk
+                   cycle
+                end if
                 if (z >= g_accn_min .and. z <= merge(nz, g_accn_max, g_accn_max == 0) .and. &
                      x >= g_accn_min_x .and. x <= merge(nx, g_accn_max_x, g_accn_max_x == 0) .and. &
                      y >= g_accn_min_y .and. y <= merge(ny, g_accn_max_y, g_accn_max_y == 0)) &

@@ -40,6 +40,8 @@ static const int cx[19] = {1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 
 static const int cy[19] = {0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0};
 static const int cz[19] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, 0};
 static const int gdeg[19] = {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+/* opposite vector, code/lb3d_lattice.F90:173-177 (0-based) */
+static const int bounce_of[19] = {1, 0, 3, 2, 5, 4, 11, 10, 13, 12, 7, 6, 9, 8, 17, 16, 15, 14, 18};
 /* float literals widened to double, e.g. dble(10.e-9) in code/lb3d_collision.F90:586 */
 #define CLAMP_RHO 9.99999993922529e-09
 #define CLAMP_SURF 9.999999747378752e-05
@@ -62,6 +64,7 @@ struct or_state {
 #define OFF_U(c) (24 * (c) + 20)
 #define OFF_DA 72
 #define OFF_DB 75
+#define OFF_RC(c) (24 * (c) + 23) /* rock_colour_r / rock_colour_b */
 
 static inline int off_rock(const or_state *o) { return o->p.ncomp == 3 ? 78 : 48; }
 static inline size_t sidx(const or_state *o, int x, int y, int z)
@@ -457,6 +460,26 @@ void or_bc_apply(or_state *o, int stage)
     }
 }
 
+void or_init_rock(or_state *o, const double *obs, double threshold)
+{
+    int nx = o->p.nx, ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
+    const int ro = off_rock(o);
+    for (int z = 1; z <= nz; z++)
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++) {
+                double *S = site(o, x, y, z);
+                size_t g = (size_t)(x - 1) + (size_t)nx * ((size_t)(y - 1) + (size_t)ny * (size_t)(z - 1));
+                S[ro] = obs[g] > threshold ? 1.0 : 0.0;
+                if (S[ro] == 0.0) continue;
+                for (int c = 0; c < C; c++)
+                    for (int s = 0; s < 19; s++) S[OFF_N(c) + s] = 0.0;
+                if (C == 3)
+                    for (int k = 0; k < 3; k++) S[OFF_DA + k] = S[OFF_DB + k] = 0.0;
+                if (o->p.rock_colour_r > 0) S[OFF_RC(0)] = o->p.rock_colour_r;
+                if (o->p.rock_colour_b > 0) S[OFF_RC(1)] = o->p.rock_colour_b;
+            }
+}
+
 /* The halo as the pre-advection pass builds it from the current populations,
  * without advancing the Lees-Edwards offset (le_halo_exchange with the
  * current lelo/lefrac; periodic copies otherwise). */
@@ -508,16 +531,32 @@ void or_stress(const or_state *o, int minx, int maxx, double out[2])
 void or_advection(or_state *o)
 {
     int nx = o->p.nx, ny = o->p.ny, nz = o->p.nz, C = o->p.ncomp;
+    const int ro = off_rock(o);
     memcpy(o->buf, o->N, sizeof(double) * (size_t)o->lx * o->ly * o->lz * o->sd);
-    #pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {
                 double *S = site(o, x, y, z);
+                const double *B0 = o->buf + sidx(o, x, y, z);
                 if (C == 3) S[OFF_DA] = S[OFF_DA + 1] = S[OFF_DA + 2] = 0.0;
+                /* half-way bounce-back, code/lb3d_bc_bounce.F90:53-90: a fluid
+                 * site whose neighbour x+c_s is rock gets its own pre-advection
+                 * n_s back as n_bounce(s) (ternary: plus n_s g_s db(x+c_s) in da) */
+                if (B0[ro] == 0.0)
+                    for (int s = 0; s < 18; s++) {
+                        const double *Ba = o->buf + sidx(o, x + cx[s], y + cy[s], z + cz[s]);
+                        if (Ba[ro] == 0.0) continue;
+                        for (int c = 0; c < C; c++) S[OFF_N(c) + bounce_of[s]] = B0[OFF_N(c) + s];
+                        if (C == 3)
+                            for (int k = 0; k < 3; k++)
+                                S[OFF_DA + k] = S[OFF_DA + k] + (B0[OFF_N(2) + s] * (double)gdeg[s]) * Ba[OFF_DB + k];
+                    }
+                /* pull from fluid neighbours into fluid sites, :118-151 */
                 for (int s = 0; s < 18; s++) {
                     const double *Bs =
                         o->buf + sidx(o, x - cx[s], y - cy[s], z - cz[s]);
+                    if (B0[ro] != 0.0 || Bs[ro] != 0.0) continue;
                     for (int c = 0; c < C; c++) S[OFF_N(c) + s] = Bs[OFF_N(c) + s];
                     if (C == 3) {
                         double ng = Bs[OFF_N(2) + s] * (double)gdeg[s];
@@ -526,7 +565,6 @@ void or_advection(or_state *o)
                     }
                 }
                 if (C == 3) {
-                    const double *B0 = o->buf + sidx(o, x, y, z);
                     double ng = B0[OFF_N(2) + 18] * (double)gdeg[18];
                     for (int k = 0; k < 3; k++) S[OFF_DA + k] = S[OFF_DA + k] + ng * B0[OFF_DB + k];
                 }
@@ -583,28 +621,41 @@ static void dipole_collision(or_state *o, int x, int y, int z)
  */
 static inline double get_psi(double rho) { return 1.0 - exp(-rho); }
 
-/* binary, :127-210 (no rock: the wall terms are zero) */
+/* binary, :127-210; a rock neighbour contributes through the wall couplings
+ * with psi(rho + rock colour) (:167-183) */
 static void sc_force_binary(const or_state *o, int x, int y, int z, double f_r[3], double f_b[3])
 {
-    double f_br[3] = {0, 0, 0}, f_rb[3] = {0, 0, 0};
+    double f_br[3] = {0, 0, 0}, f_rb[3] = {0, 0, 0}, f_wr[3] = {0, 0, 0}, f_wb[3] = {0, 0, 0};
+    const int ro = off_rock(o);
     for (int s = 0; s < 18; s++) {
         const double *Sa = csite(o, x + cx[s], y + cy[s], z + cz[s]);
-        double psi_r = get_psi(Sa[OFF_RHO(0)]);
-        double psi_b = get_psi(Sa[OFF_RHO(1)]);
-        double green_r = psi_b * (double)gdeg[s];
-        double green_b = psi_r * (double)gdeg[s];
-        f_rb[0] = f_rb[0] + green_r * (double)cx[s];
-        f_rb[1] = f_rb[1] + green_r * (double)cy[s];
-        f_rb[2] = f_rb[2] + green_r * (double)cz[s];
-        f_br[0] = f_br[0] + green_b * (double)cx[s];
-        f_br[1] = f_br[1] + green_b * (double)cy[s];
-        f_br[2] = f_br[2] + green_b * (double)cz[s];
+        const double c3[3] = {(double)cx[s], (double)cy[s], (double)cz[s]};
+        if (Sa[ro] != 0.0) {
+            double psi_r = get_psi(Sa[OFF_RHO(0)] + Sa[OFF_RC(0)]);
+            double psi_b = get_psi(Sa[OFF_RHO(1)] + Sa[OFF_RC(1)]);
+            double green_r = psi_b * (double)gdeg[s];
+            double green_b = psi_r * (double)gdeg[s];
+            for (int k = 0; k < 3; k++) {
+                f_wb[k] = f_wb[k] + green_r * c3[k];
+                f_wr[k] = f_wr[k] + green_b * c3[k];
+            }
+        } else {
+            double psi_r = get_psi(Sa[OFF_RHO(0)]);
+            double psi_b = get_psi(Sa[OFF_RHO(1)]);
+            double green_r = psi_b * (double)gdeg[s];
+            double green_b = psi_r * (double)gdeg[s];
+            for (int k = 0; k < 3; k++) {
+                f_rb[k] = f_rb[k] + green_r * c3[k];
+                f_br[k] = f_br[k] + green_b * c3[k];
+            }
+        }
     }
     const double *S0 = csite(o, x, y, z);
     double pb = get_psi(S0[OFF_RHO(1)]), pr = get_psi(S0[OFF_RHO(0)]);
     for (int k = 0; k < 3; k++) {
-        f_b[k] = (-(o->p.g_br * f_br[k])) * pb;
-        f_r[k] = (-(o->p.g_br * f_rb[k])) * pr;
+        double wb = -(o->p.g_wb * f_wb[k]), wr = -(o->p.g_wr * f_wr[k]);
+        f_b[k] = ((-(o->p.g_br * f_br[k])) + (wb / o->p.tau[1]) * o->p.tau_wb) * pb;
+        f_r[k] = ((-(o->p.g_br * f_rb[k])) + (wr / o->p.tau[0]) * o->p.tau_wr) * pr;
     }
 }
 
@@ -693,8 +744,9 @@ void or_collision(or_state *o)
     for (int z = 1; z <= nz; z++)
         for (int y = 1; y <= ny; y++)
             for (int x = 1; x <= nx; x++) {
-                double f[3][3];
-                or_sc_force(o, x, y, z, f);
+                double f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                /* only fluid sites feel the Shan-Chen force, :160-172 */
+                if (csite(o, x, y, z)[off_rock(o)] == 0.0) or_sc_force(o, x, y, z, f);
                 size_t i = (size_t)(x - 1) + (size_t)nx * ((size_t)(y - 1) + (size_t)ny * (size_t)(z - 1));
                 for (int c = 0; c < 3; c++)
                     for (int k = 0; k < 3; k++) o->force[(c * ns + i) * 3 + k] = f[c][k];
@@ -776,6 +828,7 @@ void or_sanity(const or_state *o, double mass[3], double mom[3], int64_t counts[
         for (int j = 1; j <= o->p.ny; j++)
             for (int k = 1; k <= o->p.nz; k++) {
                 const double *S = csite(o, i, j, k);
+                if (S[off_rock(o)] != 0.0) continue; /* no_rock, :134 */
                 int isnan_site = 0, isneg = 0, fast = 0;
                 for (int c = 0; c < C; c++)
                     for (int s = 0; s < 19; s++) {
@@ -833,6 +886,7 @@ void or_sanity_full(const or_state *o, or_sanity_report *r)
         for (int j = 1; j <= o->p.ny; j++)
             for (int k = 1; k <= o->p.nz; k++) {
                 const double *S = csite(o, i, j, k);
+                if (S[off_rock(o)] != 0.0) continue; /* no_rock, :134 */
                 r->fluid_sites += 1;
                 int isnan_site = 0, isneg = 0, fast = 0;
                 for (int c = 0; c < C; c++)
@@ -907,11 +961,11 @@ static double mom_of(const double *S, int c, const int *cv)
 
 void or_dump_od(const or_state *o, double *out)
 { /* dump_od, :1100-1130 */
-    FOR_INTERIOR(o) out[_i] = o->p.amass[0] * rho_of(csite(o, x, y, z), 0);
+    FOR_INTERIOR(o) out[_i] = csite(o, x, y, z)[off_rock(o)] == 0.0 ? o->p.amass[0] * rho_of(csite(o, x, y, z), 0) : 0.0;
 }
 void or_dump_wd(const or_state *o, double *out)
 { /* dump_wd, :1136-1166 */
-    FOR_INTERIOR(o) out[_i] = o->p.amass[1] * rho_of(csite(o, x, y, z), 1);
+    FOR_INTERIOR(o) out[_i] = csite(o, x, y, z)[off_rock(o)] == 0.0 ? o->p.amass[1] * rho_of(csite(o, x, y, z), 1) : 0.0;
 }
 void or_dump_sur(const or_state *o, double *out)
 { /* dump_sur, :922-948 */
@@ -931,6 +985,10 @@ void or_dump_vel(const or_state *o, double *out)
     FOR_INTERIOR(o)
     {
         const double *S = csite(o, x, y, z);
+        if (S[off_rock(o)] != 0.0) { /* rock sites dump 0, :1031 */
+            out[_i] = 0.0;
+            continue;
+        }
         double v = (mom_of(S, 0, cz) * o->p.amass[0]) * o->omega[0];
         double den = (rho_of(S, 0) * o->p.amass[0]) * o->omega[0];
         for (int c = 1; c < C; c++) {
@@ -960,8 +1018,9 @@ void or_dump_arr(const or_state *o0, double *out)
     FOR_INTERIOR(o)
     {
         const double *S = csite(o, x, y, z);
-        double f[3][3];
-        or_sc_force(o, x, y, z, f);
+        const int fluid = S[off_rock(o)] == 0.0;
+        double f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (fluid) or_sc_force(o, x, y, z, f);
         double den = rho_of(S, 0) * o->p.amass[0];
         for (int c = 1; c < C; c++) den = den + rho_of(S, c) * o->p.amass[c];
         /* max(10.e-9, real(...)): single-precision literal and conversion */
@@ -977,7 +1036,7 @@ void or_dump_arr(const or_state *o0, double *out)
             }
             v = v / dd;
             /* external force term inside the body-force box, :1428-1442 */
-            if (applyforce(o, x, y, z)) v = v - o->p.g_accn[k] / 2.0;
+            if (applyforce(o, x, y, z) && fluid) v = v - o->p.g_accn[k] / 2.0;
             out[3 * _i + k] = v;
         }
     }

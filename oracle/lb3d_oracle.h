@@ -40,6 +40,10 @@ typedef struct or_params {
     double g_accn[3];        /* g_accn_x, g_accn_y, g_accn */
     int32_t g_accn_min[3];   /* body-force box, global 1-based inclusive (x,y,z) */
     int32_t g_accn_max[3];
+    /* walls (code/lb3d_config.F90:181-183, 260-261): wall couplings and
+     * relaxation weights, rock colours of lbe_init_rock_colour */
+    double g_wr, g_wb, tau_wr, tau_wb;
+    double rock_colour_r, rock_colour_b;
 } or_params;
 
 typedef struct or_state or_state;
@@ -65,6 +69,13 @@ void or_collision(or_state *o);              /* stage 9 */
 /* one timestep nt -> nt+1 (code/lb3d_time_loop.F90:58-92) and n of them */
 void or_time_step(or_state *o);
 void or_run(or_state *o, int nsteps);
+
+/* obstacles (lbe_init_rock, code/lb3d_ic_rock.F90:50-77 with read_rock_xdrf_par,
+ * code/lb3d_io_xdrf.F90:400-470, and lbe_init_rock_colour, :146-189): a site is
+ * rock where obs (interior, x fastest) exceeds threshold, fluid elsewhere; rock
+ * sites lose their populations (and ternary da, db) and take the rock colours
+ * that are positive.  Call before or_bc_init. */
+void or_init_rock(or_state *o, const double *obs, double threshold);
 
 /* the pre-advection halo of the current state, offset not advanced */
 void or_halo_refresh(or_state *o);

@@ -31,6 +31,9 @@ class OrParams(ctypes.Structure):
         ("shear_sum", ctypes.c_double),
         ("g_accn", ctypes.c_double * 3),
         ("g_accn_min", ctypes.c_int32 * 3), ("g_accn_max", ctypes.c_int32 * 3),
+        ("g_wr", ctypes.c_double), ("g_wb", ctypes.c_double),
+        ("tau_wr", ctypes.c_double), ("tau_wb", ctypes.c_double),
+        ("rock_colour_r", ctypes.c_double), ("rock_colour_b", ctypes.c_double),
     ]
 
 
@@ -92,6 +95,7 @@ def lib():
         L.or_halo_refresh.argtypes = [vp]
         L.or_stress.argtypes = [vp, ip, ip, dp]
         L.or_sanity_full.argtypes = [vp, ctypes.POINTER(OrSanity)]
+        L.or_init_rock.argtypes = [vp, dp, ctypes.c_double]
         L.or_threads.restype = ip
         _lib = L
     return _lib
@@ -111,6 +115,7 @@ DEFAULTS = dict(
     amass=(1.0, 1.0, 1.0), tau=(1.0, 1.0, 1.0), tau_d=1.0, beta=10.0, d_0=1.0,
     g_br=0.08, g_bs=-0.006, g_ss=-0.003, shear_u=0.0, shear_omega=0.0, shear_sum=0.0,
     g_accn=(0.0, 0.0, 0.0), g_accn_min=(0, 0, 0), g_accn_max=None,
+    g_wr=0.0, g_wb=0.0, tau_wr=1.0, tau_wb=1.0, rock_colour_r=0.0, rock_colour_b=0.0,
 )
 
 
@@ -122,7 +127,7 @@ def make_params(nx, ny, nz, **kw):
     for k in ("ncomp", "bdist", "lees_edwards", "restore"):
         setattr(p, k, int(d[k]))
     for k in ("tau_d", "beta", "d_0", "g_br", "g_bs", "g_ss", "shear_u", "shear_omega",
-              "shear_sum"):
+              "shear_sum", "g_wr", "g_wb", "tau_wr", "tau_wb", "rock_colour_r", "rock_colour_b"):
         setattr(p, k, float(d[k]))
     p.amass[:] = [float(v) for v in d["amass"]]
     p.tau[:] = [float(v) for v in d["tau"]]
@@ -196,6 +201,13 @@ class Oracle:
         lib().or_le_state(self._h, ctypes.byref(s), ctypes.byref(lo), ctypes.byref(f),
                           ctypes.byref(u))
         return s.value, lo.value, f.value, u.value
+
+    def init_rock(self, obs, threshold=0.0):
+        """Obstacles from a per-site field obs (nz, ny, nx): rock where obs > threshold
+        (lbe_init_rock, code/lb3d_ic_rock.F90:50-189)."""
+        a = np.ascontiguousarray(obs, dtype=np.float64)
+        assert a.shape == (self.nz, self.ny, self.nx)
+        lib().or_init_rock(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(threshold))
 
     def halo_refresh(self):
         """Rebuild the halo as the pre-advection pass would, offset not advanced."""

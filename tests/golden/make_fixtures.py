@@ -1,7 +1,7 @@
 """Write tests/golden/reference_goldens.npz from the reference's own golden tests.
 
 Reads, as data only, the checkpoint inputs and the float32 XDR dumps that the
-reference's regression tests hold (test/{gyroid,leesEdwards,bodyForce}/,
+reference's regression tests hold (test/{gyroid,leesEdwards,bodyForce,wettingWall}/,
 run by testLB3D.sh), and stores them unchanged in one compressed npz so that
 tests run where the reference tree is absent.  Formats:
   checkpoint_*_p000000.xdr  big-endian FP64 records, x outer .. z inner,
@@ -21,6 +21,7 @@ TESTS = {
     "gyroid": (64, ["od", "wd", "sur", "dir", "vel", "arr"]),
     "leesEdwards": (39, ["od", "wd", "vel", "arr"]),
     "bodyForce": (64, ["od", "wd", "sur", "dir", "vel", "arr"]),
+    "wettingWall": (39, ["od", "wd", "vel", "arr"]),
 }
 
 
@@ -39,6 +40,10 @@ def main(ref):
             d = np.fromfile(os.path.join(base, "reference", f"{o}_{name}.reference-t00000010.xdr"),
                             dtype=">f4").astype("<f4")
             out[f"{name}/{o}"] = d.reshape(16, 16, 16, 3) if o in ("dir", "arr") else d.reshape(16, 16, 16)
+    # the obstacle file of test/wettingWall: one big-endian float32 per site,
+    # z outer .. x inner (read_rock_xdrf_par, code/lb3d_io_xdrf.F90:400-470)
+    wall = np.fromfile(os.path.join(ref, "test", "wettingWall", "wettingWall.wall.xdr"), dtype=">f4")
+    out["wettingWall/wall"] = wall.astype("<f4").reshape(16, 16, 16)
     here = os.path.dirname(os.path.abspath(__file__))
     # the namelist input files of the same tests (data, read by the Fortran host)
     os.makedirs(os.path.join(here, "inputs"), exist_ok=True)

@@ -48,7 +48,7 @@ def test_too_large_lattice_rejected():
 
 def test_params_struct_layout():
     # the Fortran bind(C) type and the C struct must agree on this size
-    assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3
+    assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3 + 8 * 6
 
 
 def test_sanity_struct_layout():

@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = {"gyroid": (3, ["od", "wd", "sur", "dir", "vel"]),
          "leesEdwards": (2, ["od", "wd", "vel"]),
-         "bodyForce": (3, ["od", "wd", "sur", "dir", "vel"])}
+         "bodyForce": (3, ["od", "wd", "sur", "dir", "vel"]),
+         "wettingWall": (2, ["od", "wd", "vel"])}
 
 
 def write_checkpoint(dirname, name, gold):
@@ -29,6 +30,9 @@ def write_checkpoint(dirname, name, gold):
     topo = np.array([1, 1, 1, 0, 0, 0], dtype=">i4").tobytes() + gold[f"{name}/shear_sum"].astype(">f8").tobytes()
     with open(os.path.join(dirname, f"checktopo_{name}_t00000000-0000000000.xdr"), "wb") as f:
         f.write(topo)
+    if f"{name}/wall" in gold:
+        # the obstacle file next to the input (obs_folder = '.')
+        gold[f"{name}/wall"].astype(">f4").tofile(os.path.join(os.path.dirname(dirname), f"{name}.wall.xdr"))
 
 
 @pytest.mark.parametrize("name", list(CASES))

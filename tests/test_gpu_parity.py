@@ -149,7 +149,7 @@ def test_stage_order_is_enforced():
     assert g.nt == 1
 
 
-@pytest.mark.parametrize("name", ["gyroid", "leesEdwards", "bodyForce"])
+@pytest.mark.parametrize("name", ["gyroid", "leesEdwards", "bodyForce", "wettingWall"])
 def test_reference_golden_tests_on_gpu(name):
     """Restore the reference test's checkpoint on the GPU, run its 10 steps,
     and reproduce its float32 dumps exactly (od, wd, sur, dir, vel, arr)."""
@@ -160,6 +160,8 @@ def test_reference_golden_tests_on_gpu(name):
     le = kw.pop("lees_edwards")
     o = Oracle(16, 16, 16, restore=1, lees_edwards=le, shear_sum=float(gold[f"{name}/shear_sum"][0]), **kw)
     o.load_checkpoint_array(gold[f"{name}/checkpoint"])
+    if f"{name}/wall" in gold:
+        o.init_rock(gold[f"{name}/wall"], 0.0)  # the walls travel with rock_state in the sites
     g = Lattice(16, 16, 16, restore=1, boundary=LB3D_BC_LEES_EDWARDS if le else 0,
                 shear_sum=float(gold[f"{name}/shear_sum"][0]), **kw)
     g.upload_sites(o.N)
@@ -174,6 +176,43 @@ def test_reference_golden_tests_on_gpu(name):
     for ob in obs:
         mine = chk.dump(ob).astype(np.float32)
         np.testing.assert_array_equal(mine, gold[f"{name}/{ob}"], err_msg=f"{name}/{ob}")
+
+
+def test_walls_fp32_close_to_fp64_and_rules():
+    """The wetting-wall case in the FP32 form tracks FP64; walls are refused
+    where they are not supported (ternary, Lees-Edwards, the batch) and the
+    one-pass plans are unavailable with them."""
+    from tests.test_oracle_golden import CASES
+    gold = np.load(GOLD)
+    kw = dict(CASES["wettingWall"])
+    kw.pop("obs")
+    kw.pop("lees_edwards")
+    o = Oracle(16, 16, 16, **kw)
+    o.load_checkpoint_array(gold["wettingWall/checkpoint"])
+    o.init_rock(gold["wettingWall/wall"], 0.0)
+    o.bc_init()
+    res = {}
+    for prec in (LB3D_FP64, LB3D_FP32_DEV):
+        g = Lattice(16, 16, 16, precision=prec, rho0=(0.35, 0.35, 0.35), **kw)
+        g.upload_sites(o.N)
+        g.bc_init()
+        with pytest.raises(LB3DError):
+            g.set_plan(1, 32, 4, 8)
+        g.time_loop(50)
+        res[prec] = g.populations()
+        N = g.download_sites()
+        assert (N[1:-1, 1:-1, 1:-1, 48] == o.interior()[..., 48]).all()
+        s = g.sanity()
+        assert s["fluid_sites"] == 4096 - 256 and s["nan_sites"] == 0
+    d = np.abs(res[LB3D_FP32_DEV] - res[LB3D_FP64]).max()
+    print(f"walls FP32 vs FP64 after 50 steps: {d:.3e}")
+    assert d < 1e-6
+    t = Oracle(16, 16, 16, ncomp=3)
+    t.init_rock(gold["wettingWall/wall"], 0.0)
+    for bad in (dict(ncomp=3), dict(ncomp=2, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.01)):
+        g = Lattice(16, 16, 16, **bad)
+        with pytest.raises(LB3DError):
+            g.upload_sites(t.N if bad["ncomp"] == 3 else o.N)
 
 
 def test_fp32_deviation_binary_density_error():

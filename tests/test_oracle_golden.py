@@ -24,6 +24,10 @@ CASES = {
     "bodyForce": dict(ncomp=3, lees_edwards=0, g_br=0.08, g_bs=-0.006, g_ss=0.003, beta=1.0,
                       d_0=1.0, tau_d=1.0, g_accn=(1e-5, 1e-5, 1e-5), g_accn_min=(10, 10, 10),
                       g_accn_max=(17, 17, 17), obs=["od", "wd", "sur", "dir", "vel", "arr"]),
+    # binary with a wetting wall read from wettingWall.wall.xdr (rock_colour = 1
+    # colours the rock red, g_wr = -0.1)
+    "wettingWall": dict(ncomp=2, lees_edwards=0, g_br=0.12, g_wr=-0.1, rock_colour_r=1.0, beta=1.0,
+                        obs=["od", "wd", "vel", "arr"]),
 }
 
 
@@ -37,6 +41,9 @@ def restore_case(gold, name):
     kw.pop("obs")
     o = Oracle(16, 16, 16, restore=1, shear_sum=float(gold[f"{name}/shear_sum"][0]), **kw)
     o.load_checkpoint_array(gold[f"{name}/checkpoint"])
+    if f"{name}/wall" in gold:
+        # lb3d_ic_init stage 4: the checkpoint, then lbe_init_rock (code/lb3d_ic.F90:76-97)
+        o.init_rock(gold[f"{name}/wall"], 0.0)
     o.bc_init()
     return o
 
