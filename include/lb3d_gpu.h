@@ -57,7 +57,9 @@ extern "C" {
 typedef struct lb3d_params {
     int32_t nx, ny, nz;       /* global lattice (tnx, tny, tnz); one GPU holds it all */
     int32_t ncomp;            /* 2 = binary (-DNOSURFACTANT), 3 = ternary (default build) */
-    int32_t bdist;            /* equilibrium: 0 (Luo) or 2 (third-order Taylor) */
+    int32_t bdist;            /* equilibrium form of code/lb3d_bdist.F90, 0-6: 0 Luo, 2 third-order
+                                 Taylor (the fast forms), 1 Hudong, 3 second order, 4/5 Guo with
+                                 the body-force term, 6 fourth-order Luo; batches take 0 and 2 */
     int32_t precision;        /* LB3D_FP64 or LB3D_FP32_DEV */
     int32_t boundary;         /* LB3D_BC_PERIODIC or LB3D_BC_LEES_EDWARDS (binary only) */
     int32_t restore;          /* 1: shear_sum was read from a checkpoint (code/lb3d_bc.F90:67-72) */
@@ -147,7 +149,8 @@ int lb3d_gpu_autotune(lb3d_ctx *ctx, int32_t trial_steps, char *report, int32_t 
 /* Choose the execution plan directly: fused = 0 (two-pass), 1 (one-pass
  * kernel with tile tx x ty and z-chunk lz, look-ahead plane staged in LDS),
  * 2 (the same with the look-ahead plane in registers) or 3 (no look-ahead
- * copy: the collide pulls its plane again); 1-3 binary with bdist 2 only.
+ * copy: the collide pulls its plane again); 1-3 binary with bdist 2 only,
+ * 3 also ternary (periodic, bdist 0 or 2).
  * xcd_group: tiles that one XCD (and its L2) takes in a row; 1 is the plain
  * round-robin order, 0 one contiguous run of tiles per XCD.  Results do not
  * depend on the plan. */

@@ -79,8 +79,8 @@ static int check_params(const lb3d_params *p)
     if (p->nx < 2 || p->ny < 1 || p->nz < 1)
         return fail(LB3D_EINVAL, "lattice extent must be nx>=2, ny>=1, nz>=1");
     if (p->ncomp != 2 && p->ncomp != 3) return fail(LB3D_EINVAL, "ncomp must be 2 or 3");
-    if (p->bdist != 0 && p->bdist != 2)
-        return fail(LB3D_EINVAL, "bdist must be 0 (Luo) or 2 (third-order Taylor)");
+    if (p->bdist < 0 || p->bdist > 6)
+        return fail(LB3D_EINVAL, "bdist must be 0-6 (code/lb3d_bdist.F90; 7 computes nothing there)");
     if (p->precision != LB3D_FP64 && p->precision != LB3D_FP32_DEV)
         return fail(LB3D_EINVAL, "precision must be LB3D_FP64 or LB3D_FP32_DEV");
     if (p->boundary != LB3D_BC_PERIODIC && p->boundary != LB3D_BC_LEES_EDWARDS)

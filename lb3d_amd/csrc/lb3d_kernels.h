@@ -23,6 +23,10 @@
 
 namespace lb3d {
 
+// collide kernels are compiled per equilibrium form: bdist 0 and 2 inline,
+// every other form (1, 3, 4, 5, 6) through boltz_general at run time
+constexpr int BD_GENERAL = 9;
+
 constexpr int NAUX2 = 4;
 constexpr int NAUX3 = 9;
 
@@ -495,10 +499,12 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
     R rhof[C];  // full densities
 #pragma unroll
     for (int c = 0; c < C; c++) rhof[c] = Dev ? P.rho0[c] + rho[c] : rho[c];
+    bool inbox = false;
     if (P.body) {
         int gx = x + 1, gy = y + 1, gz = z + 1;
-        if (gz >= P.bmin[2] && gz <= P.bmax[2] && gx >= P.bmin[0] && gx <= P.bmax[0] &&
-            gy >= P.bmin[1] && gy <= P.bmax[1]) {
+        inbox = gz >= P.bmin[2] && gz <= P.bmax[2] && gx >= P.bmin[0] && gx <= P.bmax[0] && gy >= P.bmin[1] &&
+                gy <= P.bmax[1];
+        if (inbox) {
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 R mass = P.amass[c] * rhof[c];
@@ -528,8 +534,25 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
         R du1 = (tp * F[c][1]) / P.amass[c];
         R du2 = (tp * F[c][2]) / P.amass[c];
         R b[19];
+        if constexpr (BD == BD_GENERAL) {
+            // the other equilibrium forms; Guo's dd = (tau - 1/2) g_accn inside the
+            // body-force box (lb3d_get_guo_force, code/lb3d_collision.F90:505-519),
+            // 0 outside it (where the reference leaves dd unset)
+            R dd0 = R(0), dd1 = R(0), dd2 = R(0);
+            if (inbox) {
+                R tm = P.tau[c] - R(0.5);
+                dd0 = tm * P.g_accn[0];
+                dd1 = tm * P.g_accn[1];
+                dd2 = tm * P.g_accn[2];
+            }
+            boltz_general<R>(P.bdist, ux, uy, uz, du0, du1, du2, dd0, dd1, dd2, b);
+            if constexpr (Dev) {
+#pragma unroll
+                for (int i = 0; i < 19; i++) b[i] = b[i] - (i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0));
+            }
+        }
         if constexpr (Dev) {
-            boltz_dev(BD, ux, uy, uz, du0, du1, du2, b);
+            if constexpr (BD != BD_GENERAL) boltz_dev(BD, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 // deviation of rho*b_i from w_i*rho0: w_i*drho + rho*(b_i - w_i)
@@ -539,7 +562,7 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
                 st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - deq));
             }
         } else {
-            boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
+            if constexpr (BD != BD_GENERAL) boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
 #pragma unroll
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];

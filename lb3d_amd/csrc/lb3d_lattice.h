@@ -55,6 +55,101 @@ struct LEState {
     int nt;
 };
 
+// The other equilibrium forms of code/lb3d_bdist.F90 (COMPAT_BDIST): 3 second
+// order in (u, du) (:372-391), 4 and 5 Guo's second and fourth order with the
+// forcing term dd (:392-455), 6 the fourth-order Luo form (:464-482), 1 the
+// iterated exponential (Hudong) form (:237-300).  Integer powers are products,
+// x**4 as (x*x)*(x*x); divisions by T are divisions, as written.
+template <typename R>
+__device__ __forceinline__ void boltz_general(int bdist, R ux, R uy, R uz, R dux, R duy, R duz, R ddx, R ddy,
+                                              R ddz, R b[19])
+{
+    const R T = R(1.0 / 3.0), T2 = T * T, T3 = T2 * T;
+    const R w = R(1.0 / 36.0), w0 = R(1.0 / 3.0);
+    if (bdist == 3) {
+        R uke = R(0.5) * (ux * ux + uy * uy + uz * uz);
+        R udotdu = ux * dux + uy * duy + uz * duz;
+        for (int s = 0; s < 18; s++) {
+            R cu = R(CX[s]) * ux + R(CY[s]) * uy + R(CZ[s]) * uz;
+            R cdu = R(CX[s]) * dux + R(CY[s]) * duy + R(CZ[s]) * duz;
+            R q = cu / T;
+            b[s] = w * (R(1.0) + q + R(0.5) * (q * q) - uke / T + cdu / T + (cu * cdu) / T2 - udotdu / T);
+        }
+        b[18] = w0 * (R(1.0) - uke / T - udotdu / T);
+        return;
+    }
+    if (bdist == 4 || bdist == 5 || bdist == 6) {
+        R tux = ux + dux, tuy = uy + duy, tuz = uz + duz;
+        R uke = R(0.5) * (tux * tux + tuy * tuy + tuz * tuz);
+        R dke = bdist == 6 ? R(0.0) : R(0.5) * (ddx * ddx + ddy * ddy + ddz * ddz);
+        R t1x = bdist == 5 ? tux - ddx : tux, t1y = bdist == 5 ? tuy - ddy : tuy, t1z = bdist == 5 ? tuz - ddz : tuz;
+        R u1ke = R(0.5) * (t1x * t1x + t1y * t1y + t1z * t1z);
+        for (int s = 0; s < 18; s++) {
+            R cu = R(CX[s]) * tux + R(CY[s]) * tuy + R(CZ[s]) * tuz;
+            R q = cu / T;
+            if (bdist == 6) {
+                R q2 = q * q, h = uke / T;
+                b[s] = w * (R(1.0) + q + R(0.5) * q2 - uke / T - (uke * cu) / T2 + (q2 * q) / R(6.0) +
+                            R(0.5) * (h * h) - (R(0.5) * ((cu * cu) * uke)) / T3 + (q2 * q2) / R(24.0));
+                continue;
+            }
+            R cd = R(CX[s]) * ddx + R(CY[s]) * ddy + R(CZ[s]) * ddz;
+            R r = cd / T;
+            R v = R(1.0) + q + R(0.5) * (q * q - r * r) - (uke - dke) / T;
+            if (bdist == 5) {
+                R c1 = R(CX[s]) * t1x + R(CY[s]) * t1y + R(CZ[s]) * t1z;
+                R q1 = c1 / T, q12 = q1 * q1, h1 = u1ke / T;
+                v = v - (uke * c1) / T2 + (q12 * q1) / R(6.0) + R(0.5) * (h1 * h1) -
+                    (R(0.5) * ((c1 * c1) * u1ke)) / T3 + (q12 * q12) / R(24.0);
+            }
+            b[s] = w * v;
+        }
+        R h = uke / T;
+        if (bdist == 4)
+            b[18] = w0 * (R(1.0) - (uke - dke) / T);
+        else if (bdist == 5)
+            b[18] = w0 * (R(1.0) - (uke - dke) / T + R(0.5) * (h * h));
+        else
+            b[18] = w0 * (R(1.0) - uke / T + R(0.5) * (h * h));
+        return;
+    }
+    // bdist == 1
+    R uxf = ux + dux, uyf = uy + duy, uzf = uz + duz;
+    R uke_eq = R(0.5) * (uxf * uxf + uyf * uyf + uzf * uzf);
+    R u_mag = sqrt(R(2.0) * uke_eq);
+    int ifactor = (int)(R(200.0) * u_mag);
+    R n_iter = R(ifactor > 2 ? ifactor : 2);
+    R delta = sqrt(R(1.0) / n_iter);
+    R uxe = uxf, uye = uyf, uze = uzf, rho_eq = R(1.0);
+    for (int it = 1; it <= (int)n_iter; it++) {
+        R ex[3] = {exp(uxe / T), exp(uye / T), exp(uze / T)};
+        R expek = R(1.0) / exp(uke_eq / T);
+        for (int s = 0; s < 18; s++) {
+            const int cv[3] = {CX[s], CY[s], CZ[s]};
+            R a = R(1.0);
+            for (int k = 0; k < 3; k++) {
+                R pk = cv[k] == 1 ? ex[k] : (cv[k] == -1 ? R(1.0) / ex[k] : R(1.0));
+                a = k == 0 ? pk : a * pk;
+            }
+            b[s] = ((rho_eq * w) * a) * expek;
+        }
+        b[18] = (rho_eq * w0) * expek;
+        R r1 = R(0.0), px = R(0.0), py = R(0.0), pz = R(0.0);
+        for (int s = 0; s < 18; s++) {
+            r1 = r1 + b[s];
+            px = px + b[s] * R(CX[s]);
+            py = py + b[s] * R(CY[s]);
+            pz = pz + b[s] * R(CZ[s]);
+        }
+        R rho_tt = r1 + b[18];
+        rho_eq = rho_eq / (R(1.0) + (delta * (rho_tt - R(1.0))) / rho_eq);
+        uxe = uxe + delta * (uxf - px);
+        uye = uye + delta * (uyf - py);
+        uze = uze + delta * (uzf - pz);
+        uke_eq = R(0.5) * (uxe * uxe + uye * uye + uze * uze);
+    }
+}
+
 // Equilibrium distribution, code/lb3d_bdist.F90:175-370 (COMPAT_BDIST):
 // bdist 0 = Luo (:214-234), 2 = third-order Taylor (:301-370), in the
 // reference's operation order.  b[i] are the weights w_i * (...), so that
@@ -65,6 +160,10 @@ __device__ __forceinline__ void boltz_dist(int bdist, R ux, R uy, R uz, R dux, R
 {
     const R holdit = R(3.0);  // 1/T with T = 1/3
     const R w = R(1.0 / 36.0);
+    if (bdist != 0 && bdist != 2) {
+        boltz_general<R>(bdist, ux, uy, uz, dux, duy, duz, R(0), R(0), R(0), b);
+        return;
+    }
     if (bdist == 0) {
         R uke = R(0.5) * (ux * ux + uy * uy + uz * uz);
         R udotdu = ux * dux + uy * duy + uz * duz;

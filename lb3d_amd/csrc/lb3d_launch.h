@@ -291,8 +291,10 @@ class LauncherT : public Launcher {
     {
         if (P.bdist == 0)
             collide_bd<0>(b, s);
-        else
+        else if (P.bdist == 2)
             collide_bd<2>(b, s);
+        else
+            collide_bd<BD_GENERAL>(b, s);
     }
     void le_faces(const DevBufs &b, bool post, hipStream_t s) override
     {
@@ -378,6 +380,7 @@ class LauncherT : public Launcher {
         if (C == 3) {
             // ternary: the reload form only, periodic lattices, nz >= 3 for the +-2 plane wrap
             if (le || lz < 1 || st != 2 || P.nz < 3 || P.nx < 3 || P.ny < 3) return false;
+            if (P.bdist != 0 && P.bdist != 2) return false;
             if (tx == 32 && ty == 4) return fused3_t<32, 4>(b, lz, s, launch);
             if (tx == 32 && ty == 8) return fused3_t<32, 8>(b, lz, s, launch);
             if (tx == 64 && ty == 4) return fused3_t<64, 4>(b, lz, s, launch);

@@ -149,9 +149,118 @@ void or_le_state(const or_state *o, double *shear_sum, int *lelo, double *lefrac
  * Equilibrium distribution, code/lb3d_bdist.F90:175-370 (COMPAT_BDIST build).
  * bdist 0 = Luo operator (:214-234), bdist 2 = third-order Taylor (:301-370).
  */
+/* The other equilibrium forms of code/lb3d_bdist.F90 (COMPAT_BDIST):
+ * 3 second order in (u, du) (:372-391), 4 and 5 Guo's second and fourth order
+ * with the forcing term dd (:392-455), 6 the fourth-order Luo form (:464-482),
+ * 1 the iterated exponential (Hudong) form (:237-300); 7 computes nothing in
+ * the reference and is rejected.  Integer powers are
+ * products, x**4 as (x*x)*(x*x); divisions by T are divisions, as written. */
+static void boltz_general(int bdist, double ux, double uy, double uz, double dux, double duy, double duz,
+                          double ddx, double ddy, double ddz, double *b)
+{
+    const double T = 1.0 / 3.0, T2 = T * T, T3 = T2 * T;
+    const double w = 1.0 / 36.0, w0 = 1.0 / 3.0;
+    if (bdist == 3) {
+        double uke = 0.5 * (ux * ux + uy * uy + uz * uz);
+        double udotdu = ux * dux + uy * duy + uz * duz;
+        for (int s = 0; s < 18; s++) {
+            double cu = cx[s] * ux + cy[s] * uy + cz[s] * uz;
+            double cdu = cx[s] * dux + cy[s] * duy + cz[s] * duz;
+            double q = cu / T;
+            b[s] = w * (1.0 + q + 0.5 * (q * q) - uke / T + cdu / T + (cu * cdu) / T2 - udotdu / T);
+        }
+        b[18] = w0 * (1.0 - uke / T - udotdu / T);
+        return;
+    }
+    if (bdist == 4 || bdist == 5 || bdist == 6) {
+        double tux = ux + dux, tuy = uy + duy, tuz = uz + duz;
+        double uke = 0.5 * (tux * tux + tuy * tuy + tuz * tuz);
+        double dke = bdist == 6 ? 0.0 : 0.5 * (ddx * ddx + ddy * ddy + ddz * ddz);
+        double t1x = bdist == 5 ? tux - ddx : tux, t1y = bdist == 5 ? tuy - ddy : tuy,
+               t1z = bdist == 5 ? tuz - ddz : tuz;
+        double u1ke = 0.5 * (t1x * t1x + t1y * t1y + t1z * t1z);
+        for (int s = 0; s < 18; s++) {
+            double cu = cx[s] * tux + cy[s] * tuy + cz[s] * tuz;
+            double q = cu / T;
+            if (bdist == 6) {
+                double q2 = q * q, h = uke / T;
+                b[s] = w * (1.0 + q + 0.5 * q2 - uke / T - (uke * cu) / T2 + (q2 * q) / 6.0 + 0.5 * (h * h) -
+                            (0.5 * ((cu * cu) * uke)) / T3 + (q2 * q2) / 24.0);
+                continue;
+            }
+            double cd = cx[s] * ddx + cy[s] * ddy + cz[s] * ddz;
+            double r = cd / T;
+            double v = 1.0 + q + 0.5 * (q * q - r * r) - (uke - dke) / T;
+            if (bdist == 5) {
+                double c1 = cx[s] * t1x + cy[s] * t1y + cz[s] * t1z;
+                double q1 = c1 / T, q12 = q1 * q1, h1 = u1ke / T;
+                v = v - (uke * c1) / T2 + (q12 * q1) / 6.0 + 0.5 * (h1 * h1) - (0.5 * ((c1 * c1) * u1ke)) / T3 +
+                    (q12 * q12) / 24.0;
+            }
+            b[s] = w * v;
+        }
+        double h = uke / T;
+        if (bdist == 4)
+            b[18] = w0 * (1.0 - (uke - dke) / T);
+        else if (bdist == 5)
+            b[18] = w0 * (1.0 - (uke - dke) / T + 0.5 * (h * h));
+        else
+            b[18] = w0 * (1.0 - uke / T + 0.5 * (h * h));
+        return;
+    }
+    /* bdist == 1 */
+    double uxf = ux + dux, uyf = uy + duy, uzf = uz + duz;
+    double uke_eq = 0.5 * (uxf * uxf + uyf * uyf + uzf * uzf);
+    double u_mag = sqrt(2.0 * uke_eq);
+    int ifactor = (int)(200.0 * u_mag);
+    double n_iter = (double)(ifactor > 2 ? ifactor : 2);
+    double delta = sqrt(1.0 / n_iter);
+    double uxe = uxf, uye = uyf, uze = uzf, rho_eq = 1.0;
+    for (int it = 1; it <= (int)n_iter; it++) {
+        double ex[3] = {exp(uxe / T), exp(uye / T), exp(uze / T)};
+        double expek = 1.0 / exp(uke_eq / T);
+        for (int s = 0; s < 18; s++) {
+            const int cv[3] = {cx[s], cy[s], cz[s]};
+            double a = 1.0;
+            for (int k = 0; k < 3; k++) {
+                double pk = cv[k] == 1 ? ex[k] : (cv[k] == -1 ? 1.0 / ex[k] : 1.0);
+                a = k == 0 ? pk : a * pk;
+            }
+            b[s] = ((rho_eq * w) * a) * expek;
+        }
+        b[18] = (rho_eq * w0) * expek;
+        double r1 = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+        for (int s = 0; s < 18; s++) {
+            r1 = r1 + b[s];
+            px = px + b[s] * cx[s];
+            py = py + b[s] * cy[s];
+            pz = pz + b[s] * cz[s];
+        }
+        double rho_tt = r1 + b[18];
+        rho_eq = rho_eq / (1.0 + (delta * (rho_tt - 1.0)) / rho_eq);
+        uxe = uxe + delta * (uxf - px);
+        uye = uye + delta * (uyf - py);
+        uze = uze + delta * (uzf - pz);
+        uke_eq = 0.5 * (uxe * uxe + uye * uye + uze * uze);
+    }
+}
+
+void or_boltz_dist_dd(int bdist, double ux, double uy, double uz, double dux, double duy, double duz,
+                      double ddx, double ddy, double ddz, double *boltz)
+{
+    if (bdist == 0 || bdist == 2)
+        or_boltz_dist(bdist, ux, uy, uz, dux, duy, duz, boltz);
+    else
+        boltz_general(bdist, ux, uy, uz, dux, duy, duz, ddx, ddy, ddz, boltz);
+}
+
 void or_boltz_dist(int bdist, double ux, double uy, double uz, double dux, double duy,
                    double duz, double *boltz)
 {
+    if (bdist != 0 && bdist != 2) {
+        boltz_general(bdist, ux, uy, uz, dux, duy, duz, 0.0, 0.0, 0.0, boltz);
+        return;
+    }
     const double T = 1.0 / 3.0;
     const double holdit = 1.0 / T;
     const double w_move = 1.0 / 36.0, w_rest = 1.0 / 3.0;
@@ -787,8 +896,14 @@ void or_collision(or_state *o)
                     double pon = 1.0 / (rho > CLAMP_RHO ? rho : CLAMP_RHO);
                     double du[3];
                     for (int k = 0; k < 3; k++) du[k] = ((o->p.tau[c] * pon) * F[c][k]) / o->p.amass[c];
+                    /* Guo forcing term dd = (tau - 1/2) g_accn inside the body-force box
+                     * (lb3d_get_guo_force, :505-519); outside it the reference leaves dd
+                     * unset, here it is 0 */
+                    double dd[3] = {0.0, 0.0, 0.0};
+                    if (applyforce(o, x, y, z))
+                        for (int k = 0; k < 3; k++) dd[k] = (o->p.tau[c] - 0.5) * o->p.g_accn[k];
                     double boltz[19];
-                    or_boltz_dist(o->p.bdist, ut[0], ut[1], ut[2], du[0], du[1], du[2], boltz);
+                    or_boltz_dist_dd(o->p.bdist, ut[0], ut[1], ut[2], du[0], du[1], du[2], dd[0], dd[1], dd[2], boltz);
                     double *n = S + OFF_N(c);
                     for (int s = 0; s < 19; s++) {
                         double neq = rho * boltz[s];

@@ -113,9 +113,12 @@ void or_dump_dir(const or_state *o, double *out);
 void or_dump_vel(const or_state *o, double *out);
 void or_dump_arr(const or_state *o, double *out);
 
-/* equilibrium distribution (code/lb3d_bdist.F90:175-370) */
+/* equilibrium distribution (code/lb3d_bdist.F90:175-389), bdist 0-6; the
+ * _dd form takes Guo's forcing term dd (bdist 4 and 5; 0 for or_boltz_dist) */
 void or_boltz_dist(int bdist, double ux, double uy, double uz, double dux,
                    double duy, double duz, double *boltz19);
+void or_boltz_dist_dd(int bdist, double ux, double uy, double uz, double dux, double duy, double duz,
+                      double ddx, double ddy, double ddz, double *boltz19);
 /* Shan-Chen force at one interior site from the current state */
 void or_sc_force(const or_state *o, int x, int y, int z, double f[3][3]);
 

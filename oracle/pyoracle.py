@@ -87,6 +87,7 @@ def lib():
                      "or_dump_vel", "or_dump_arr"):
             getattr(L, name).argtypes = [vp, dp]
         L.or_boltz_dist.argtypes = [ip] + [ctypes.c_double] * 6 + [dp]
+        L.or_boltz_dist_dd.argtypes = [ip] + [ctypes.c_double] * 9 + [dp]
         L.or_sc_force.argtypes = [vp, ip, ip, ip, ctypes.POINTER(ctypes.c_double * 9)]
         L.or_init_noise.argtypes = [vp, ctypes.c_uint64, dp, dp, ip]
         L.or_hash_uniform.restype = ctypes.c_double

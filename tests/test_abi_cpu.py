@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.parametrize("kw,msg", [
     (dict(ncomp=3, boundary=gpu.LB3D_BC_LEES_EDWARDS), "surfactant"),
     (dict(ncomp=4), "ncomp"),
-    (dict(bdist=1), "bdist"),
+    (dict(bdist=7), "bdist"),
     (dict(precision=7), "precision"),
     (dict(tau=(0.0, 1.0, 1.0)), "tau"),
 ])

@@ -119,6 +119,27 @@ def test_unequal_tau_amass_and_body_force():
     assert_pops_close(o, g, what="unequal tau/amass + body force")
 
 
+@pytest.mark.parametrize("bdist", [1, 3, 4, 5, 6])
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_other_equilibrium_forms(bdist, ncomp):
+    """The remaining forms of code/lb3d_bdist.F90 (1: iterated exponential,
+    3: second order, 4: Guo, 5: Guo to fourth order, 6: fourth order), with a
+    body-force box so that Guo's (tau - 1/2) g term is on inside and off
+    outside it; FP32 deviation storage follows the FP64 form."""
+    kw = dict(ncomp=ncomp, bdist=bdist, g_br=0.08, tau=(0.9, 1.2, 1.0),
+              g_accn=(2e-4, -1e-4, 3e-4), g_accn_min=(2, 3, 2), g_accn_max=(6, 8, 5))
+    o, g = pair(9, 10, 8, amp=(0.03, 0.03, 0.02), **kw)
+    o.run(12)
+    g.time_loop(12)
+    assert_pops_close(o, g, what=f"bdist {bdist} ncomp {ncomp}")
+    _, g32 = pair(9, 10, 8, amp=(0.03, 0.03, 0.02), precision=LB3D_FP32_DEV, **kw)
+    g32.time_loop(12)
+    a, b = g.populations(), g32.populations()
+    rel = np.abs(a - b).max() / np.abs(a).max()
+    print(f"bdist {bdist} ncomp {ncomp}: FP32 vs FP64 {rel:.3e}")
+    assert rel < 2e-6
+
+
 def test_staged_surface_equals_graph_replay():
     """The reference's stage calls and the captured-graph loop give identical bits."""
     for kw in (dict(ncomp=3), dict(ncomp=2, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.05)):

@@ -268,3 +268,58 @@ def test_sanity_full_uniform_flow_and_ties():
     np.testing.assert_allclose(s["max_vel"][0][0], 0.08, rtol=1e-12)
     I[2, 0, 4, 24 + 3] = np.nan
     assert o.sanity_full()["nan_sites"] == 1
+
+
+def boltz_dd(bdist, u, du=(0.0, 0.0, 0.0), dd=(0.0, 0.0, 0.0)):
+    out = (ctypes.c_double * 19)()
+    lib().or_boltz_dist_dd(bdist, *[float(v) for v in u], *[float(v) for v in du], *[float(v) for v in dd], out)
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("bdist", [3, 4, 6])
+def test_other_equilibria_mass_and_momentum(bdist):
+    """bdist 3 (second order in u and du), 4 (Guo, second order), 6 (fourth
+    order): sum g b = 1 and sum g b c = u + du, g the link multiplicities (isotropy of the D3Q19 weights up to
+    the fourth moment)."""
+    u, du = np.array([0.031, -0.017, 0.022]), np.array([0.004, 0.002, -0.003])
+    b = boltz_dd(bdist, u, du, (0.01, -0.02, 0.005) if bdist == 4 else (0, 0, 0))
+    np.testing.assert_allclose(G @ b, 1.0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(C3.T @ (G * b), u + du, rtol=1e-13, atol=1e-16)
+
+
+def test_guo_second_order_stress_and_fourth_order_forms():
+    """bdist 4: sum b c c = T I + (u+du)(u+du) - dd dd (Guo et al., PRE 65
+    046308); bdist 5 with dd = 0 is bdist 6 exactly, and with dd its first
+    moment is tu + (u1ke - uke) tu1 / T with tu1 = tu - dd."""
+    u, du, dd = np.array([0.02, -0.01, 0.03]), np.array([0.001, 0.0, -0.002]), np.array([0.004, -0.006, 0.002])
+    tu = u + du
+    b4 = boltz_dd(4, u, du, dd)
+    P = (C3.T * (G * b4)) @ C3
+    np.testing.assert_allclose(P, np.eye(3) / 3.0 + np.outer(tu, tu) - np.outer(dd, dd), rtol=0, atol=1e-16)
+    np.testing.assert_array_equal(boltz_dd(5, u, du), boltz_dd(6, u, du))
+    b5 = boltz_dd(5, u, du, dd)
+    tu1 = tu - dd
+    uke, u1ke = 0.5 * tu @ tu, 0.5 * tu1 @ tu1
+    np.testing.assert_allclose(C3.T @ (G * b5), tu + (u1ke - uke) * tu1 * 3.0, rtol=1e-12, atol=1e-17)
+    # bdist 6 conserves mass to fourth order exactly
+    np.testing.assert_allclose(G @ boltz_dd(6, u, du), 1.0, rtol=0, atol=1e-15)
+
+
+def test_hudong_iterated_exponential():
+    """bdist 1 (code/lb3d_bdist.F90:237-298): b_i = rho_eq w_i exp((c_i.u_eq -
+    |u_eq|^2/2)/T), so log(b_i / w_i) is affine in c_i.  The damped iteration
+    drives the plain (unweighted, as the reference sums them) moments towards
+    (1, u + du); max(2, 200|u|) iterations of step sqrt(1/n) converge further
+    the faster the flow."""
+    w = np.array([1 / 36.0] * 18 + [1 / 3.0])
+    A = np.hstack([np.ones((19, 1)), C3])
+    errs = []
+    for m in (0.05, 0.1, 0.2, 0.3):
+        u = np.array([0.6, -0.48, 0.64]) * m
+        b = boltz_dd(1, u - 0.01, (0.01, 0.01, 0.01))
+        y = np.log(b / w)
+        coef = np.linalg.lstsq(A, y, rcond=None)[0]
+        np.testing.assert_allclose(A @ coef, y, rtol=0, atol=1e-13)
+        errs.append((abs(b.sum() - 1.0), np.abs(C3.T @ b - u).max()))
+    assert all(errs[k + 1][0] < errs[k][0] for k in range(3))
+    assert errs[-1][0] < 3e-4 and errs[-1][1] < 2e-4
