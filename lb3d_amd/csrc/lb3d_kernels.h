@@ -154,6 +154,21 @@ __device__ __forceinline__ R psi_of(R rho, R emr0)
 // (code/lb3d_analysis.F90:64-76); ternary also the advected dipole
 // da = sum_i g_i n_s,i(x-c_i) db(x-c_i) (:130-148).  The streamed
 // populations themselves are not stored: k_collide pulls them again.
+// One term of the advected dipole: the streamed surfactant population v of
+// direction i (stored form) times the dipole field at its source x - c_i.
+template <typename R, bool Dev>
+__device__ __forceinline__ void acc_da(const Phys<R> &P, const R *__restrict__ db, int i, R v, int x, int y, int z,
+                                       R &da0, R &da1, R &da2)
+{
+    R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
+    R ng = ns_ * R(GD[i]);
+    int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny), zs = wrap1(z - CZ[i], P.nz);
+    uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
+    da0 = da0 + ng * ld_plane(db, q);
+    da1 = da1 + ng * ld_plane(plane_of(db, 1u, P.ns), q);
+    da2 = da2 + ng * ld_plane(plane_of(db, 2u, P.ns), q);
+}
+
 template <typename R, bool Dev, int C, bool LE, bool Rock = false>
 __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
@@ -176,18 +191,8 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
             else
                 v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
             rho[c] = rho[c] + v * R(GD[i]);
-            if constexpr (C == 3) {
-                if (c == 2) {
-                    R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
-                    R ng = ns_ * R(GD[i]);
-                    int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny),
-                        zs = wrap1(z - CZ[i], P.nz);
-                    uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
-                    da0 = da0 + ng * ld_plane(db, q);
-                    da1 = da1 + ng * ld_plane(plane_of(db, 1u, P.ns), q);
-                    da2 = da2 + ng * ld_plane(plane_of(db, 2u, P.ns), q);
-                }
-            }
+            if constexpr (C == 3)
+                if (c == 2) acc_da<R, Dev>(P, db, i, v, x, y, z, da0, da1, da2);
         }
     }
     da[0] = da0;
@@ -473,6 +478,23 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
 // pass and the relaxation pass each call it anew (a compiler memory barrier
 // between them keeps the loads from being shared), so that no population
 // stays live in registers across the two passes.
+// Density and momentum of component c from one population v of direction i,
+// accumulated in direction order; every kernel that needs the moments of the
+// streamed state sums them through this, so all plans round alike.
+template <typename R>
+__device__ __forceinline__ void acc_moments(int i, R v, R &rho, R (&mom)[3])
+{
+    rho = rho + v * R(GD[i]);
+    R ng = v * R(GD[i]);
+    if (CX[i] != 0) mom[0] = mom[0] + ng * R(CX[i]);
+    if (CY[i] != 0) mom[1] = mom[1] + ng * R(CY[i]);
+    if (CZ[i] != 0) mom[2] = mom[2] + ng * R(CZ[i]);
+}
+
+template <typename R, bool Dev, int C, int BD, typename Get>
+__device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
+                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout);
+
 template <typename R, bool Dev, int C, int BD, bool Refetch = false, typename Get>
 __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][3], int x, int y, int z,
                                            uint32_t s, R *__restrict__ fout)
@@ -486,16 +508,18 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
 #pragma unroll
     for (int i = 0; i < 19; i++) {
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            R v = get(c, i);
-            rho[c] = rho[c] + v * R(GD[i]);
-            R ng = v * R(GD[i]);
-            if (CX[i] != 0) mom[c][0] = mom[c][0] + ng * R(CX[i]);
-            if (CY[i] != 0) mom[c][1] = mom[c][1] + ng * R(CY[i]);
-            if (CZ[i] != 0) mom[c][2] = mom[c][2] + ng * R(CZ[i]);
-        }
+        for (int c = 0; c < C; c++) acc_moments<R>(i, get(c, i), rho[c], mom[c]);
     }
     if constexpr (Refetch) asm volatile("" ::: "memory");
+    bgk_relax<R, Dev, C, BD>(P, get, rho, mom, F, x, y, z, s, fout);
+}
+
+// The relaxation of bgk_site_g from the moments (rho, mom) of the streamed
+// populations that get(c, i) returns.
+template <typename R, bool Dev, int C, int BD, typename Get>
+__device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
+                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout)
+{
     R rhof[C];  // full densities
 #pragma unroll
     for (int c = 0; c < C; c++) rhof[c] = Dev ? P.rho0[c] + rho[c] : rho[c];
@@ -832,151 +856,237 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
 // (code/lb3d_collision.F90:296-403), ternary Shan-Chen forces
 // (code/lb3d_force_shanchen.F90:215-330) and BGK, marching along z through a
 // TX x TY column tile.  The force at a site needs psi and the new dipole
-// field one site around it, and the new dipole field needs the moments one
-// site further out, so per plane the work-group computes
-//   the moments of plane k+2 on the tile + 2 sites (LDS ring of 4 planes),
-//   the dipole field of plane k+1 on the tile + 1 site (LDS ring of 3 planes),
-//   and collides plane k (pulling its populations again, an L2 hit).
+// field one site around it, and the new dipole field needs rho_b - rho_r and
+// the advected dipole one site further out.  Per plane k the work-group
+//   M  computes the moments of plane k+2 on the tile and a ring of 2 sites:
+//      each thread those of its own site, whose densities and momenta stay
+//      in its registers until the collide of that plane, and the ring sites
+//      spread over the threads;
+//   D  relaxes the dipole field of plane k+1 on the tile and a ring of 1 site;
+//   C  collides plane k from the registered moments, the forces and a
+//      second pull of its populations.
+// LDS keeps per site only what a later stage reads at that distance:
+//   tile + 2 sites, 3 planes: rho_b - rho_r and the advected dipole;
+//   tile + 1 site: psi of the 3 components (4 planes), rho_s (2 planes) and
+//   the new dipole field (3 planes).
 // The new dipole field of the tile's own sites goes to db_out; db_in (the
 // previous step's field, read for the advected dipole) is left untouched, so
 // neighbouring work-groups at other planes read consistent data.
-constexpr int FUSED3_NM = 8;  // per point: psi_r, psi_b, psi_s, rho_b - rho_r, rho_s, da_xyz
-
 template <int TX, int TY>
 struct Fused3Dims {
     static constexpr int T = TX * TY;
-    static constexpr int MX = TX + 4, MY = TY + 4, MP = MX * MY;  // moments region
-    static constexpr int DX = TX + 2, DY = TY + 2, DP = DX * DY;  // dipole region
+    static constexpr int MX = TX + 4, MY = TY + 4, MP = MX * MY;  // tile + 2 sites
+    static constexpr int DX = TX + 2, DY = TY + 2, DP = DX * DY;  // tile + 1 site
+    static constexpr int NH = MP - T;                             // ring sites of M
+    static constexpr int NHT = (NH + T - 1) / T;                  // ring sites per thread
+    template <typename R>
+    static constexpr size_t lds_bytes()
+    {
+        return (size_t)(3 * 4 * MP + (4 * 3 + 2 + 3 * 3) * DP) * sizeof(R);
+    }
+    // waves per SIMD that the LDS leaves room for (160 KiB and 4 SIMDs per
+    // CU): the register bound of the kernel
+    template <typename R>
+    static constexpr int waves()
+    {
+        int wg = (int)((160 * 1024) / lds_bytes<R>());
+        int w = wg * (T / 64) / 4;
+        return w < 1 ? 1 : (w > 8 ? 8 : w);
+    }
+    // ring site h in [0, NH) of the moments region: rows 0-1, rows MY-2 and
+    // MY-1, then columns 0, 1, MX-2, MX-1 of the rows between
+    __device__ static __forceinline__ void ring_site(int h, int &mx, int &my)
+    {
+        if (h < 2 * MX) {
+            my = h / MX;
+            mx = h - my * MX;
+        } else if (h < 4 * MX) {
+            int g = h - 2 * MX;
+            my = MY - 2 + g / MX;
+            mx = g - (g / MX) * MX;
+        } else {
+            int g = h - 4 * MX;
+            my = 2 + g / 4;
+            int c = g & 3;
+            mx = c < 2 ? c : MX - 4 + c;
+        }
+    }
 };
 
 template <typename R, bool Dev, int TX, int TY, int BD>
-__device__ __forceinline__ void fused3_tile(const Phys<R> &P, const R *__restrict__ f,
-                                            const R *__restrict__ db_in, R *__restrict__ db_out,
-                                            R *__restrict__ fout, int x0, int y0, int z0, int zend,
-                                            R (&mring)[4][FUSED3_NM][Fused3Dims<TX, TY>::MP],
-                                            R (&dring)[3][3][Fused3Dims<TX, TY>::DP])
+__global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) void k_fused3(
+    Phys<R> P, const R *__restrict__ f, const R *__restrict__ db_in, R *__restrict__ db_out,
+    R *__restrict__ fout, int lz, uint32_t xg)
 {
     using D = Fused3Dims<TX, TY>;
-    const int tid = threadIdx.x;
-    const int lx = tid % TX, ly = tid / TX;
-    const int x = x0 + lx, y = y0 + ly;
-    const bool own = x < P.nx && y < P.ny;
-    auto mslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 2) & 3; };
-    auto dslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 1) % 3; };
-
-    // moments of plane zp at the tile + 2 sites
-    auto fillM = [&](int zp) __attribute__((always_inline)) {
-        const int sl = mslot(zp);
-        const int zw = wrap1(zp, P.nz);
-        for (int p = tid; p < D::MP; p += D::T) {
-            int gx = x0 - 2 + p % D::MX, gy = y0 - 2 + p / D::MX;
-            if (gx > P.nx + 1 || gy > P.ny + 1) continue;  // beyond a partial tile's needs
-            int xw = wrap1(gx, P.nx), yw = wrap1(gy, P.ny);
-            asm volatile("" : "+v"(xw), "+v"(yw));
-            R rho[3], da[3];
-            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xw, yw, zw, rho, da);
-            mring[sl][0][p] = psi_of<Dev>(rho[0], P.emr0[0]);
-            mring[sl][1][p] = psi_of<Dev>(rho[1], P.emr0[1]);
-            mring[sl][2][p] = psi_of<Dev>(rho[2], P.emr0[2]);
-            mring[sl][3][p] = rho[1] - rho[0];
-            mring[sl][4][p] = rho[2];
-            mring[sl][5][p] = da[0];
-            mring[sl][6][p] = da[1];
-            mring[sl][7][p] = da[2];
-        }
-    };
-    // dipole field of plane zp at the tile + 1 site; own sites of planes
-    // z0..zend-1 also go to db_out
-    auto fillD = [&](int zp) __attribute__((always_inline)) {
-        const int dsl = dslot(zp);
-        const int sl0 = mslot(zp), slm = mslot(zp - 1), slp = mslot(zp + 1);
-        for (int p = tid; p < D::DP; p += D::T) {
-            int dx = p % D::DX, dy = p / D::DX;
-            int gx = x0 - 1 + dx, gy = y0 - 1 + dy;
-            if (gx > P.nx || gy > P.ny) continue;
-            const int q = (dy + 1) * D::MX + (dx + 1);
-            R out[3];
-            dipole_core<R, Dev>(
-                P, mring[sl0][5][q], mring[sl0][6][q], mring[sl0][7][q], mring[sl0][4][q],
-                [&](int i) __attribute__((always_inline)) {
-                    int sl = CZ[i] == 0 ? sl0 : (CZ[i] > 0 ? slp : slm);
-                    return mring[sl][3][q + CY[i] * D::MX + CX[i]];
-                },
-                [&](int i, int a) __attribute__((always_inline)) {
-                    int sl = CZ[i] == 0 ? sl0 : (CZ[i] > 0 ? slp : slm);
-                    return mring[sl][5 + a][q + CY[i] * D::MX + CX[i]];
-                },
-                out);
-            dring[dsl][0][p] = out[0];
-            dring[dsl][1][p] = out[1];
-            dring[dsl][2][p] = out[2];
-            if (dx >= 1 && dx <= TX && dy >= 1 && dy <= TY && gx < P.nx && gy < P.ny && zp >= z0 && zp < zend) {
-                uint32_t sg = sidx(gx, gy, zp, P.nx, P.ny);
-                st_plane(db_out, sg, out[0]);
-                st_plane(db_out + (size_t)P.ns, sg, out[1]);
-                st_plane(db_out + (size_t)2 * P.ns, sg, out[2]);
-            }
-        }
-    };
-    // prologue: moments of planes z0-2 .. z0+1, dipoles of z0-1 and z0
-    fillM(z0 - 2);
-    fillM(z0 - 1);
-    fillM(z0);
-    fillM(z0 + 1);
-    __syncthreads();
-    fillD(z0 - 1);
-    fillD(z0);
-    __syncthreads();
-    const int qm = (ly + 2) * D::MX + (lx + 2), qd = (ly + 1) * D::DX + (lx + 1);
-    for (int k = z0; k < zend; k++) {
-        fillM(k + 2);
-        __syncthreads();
-        fillD(k + 1);
-        __syncthreads();
-        if (own) {
-            const int sm0 = mslot(k), smm = mslot(k - 1), smp = mslot(k + 1);
-            const int sd0 = dslot(k), sdm = dslot(k - 1), sdp = dslot(k + 1);
-            const R dbx[3] = {dring[sd0][0][qd], dring[sd0][1][qd], dring[sd0][2][qd]};
-            const R ps_own[3] = {mring[sm0][0][qm], mring[sm0][1][qm], mring[sm0][2][qm]};
-            R F[3][3];
-            sc_forces3<R, Dev>(
-                P, dbx, ps_own,
-                [&](int i, int c) __attribute__((always_inline)) {
-                    int sl = CZ[i] == 0 ? sm0 : (CZ[i] > 0 ? smp : smm);
-                    return mring[sl][c][qm + CY[i] * D::MX + CX[i]];
-                },
-                [&](int i, int a) __attribute__((always_inline)) {
-                    int sl = CZ[i] == 0 ? sd0 : (CZ[i] > 0 ? sdp : sdm);
-                    return dring[sl][a][qd + CY[i] * D::DX + CX[i]];
-                },
-                F);
-            bgk_site_g<R, Dev, 3, BD, true>(
-                P,
-                [&](int c, int i) __attribute__((always_inline)) {
-                    return pull<R, 3, false>(f, nullptr, P, c, i, x, y, k);
-                },
-                F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
-        }
-        __syncthreads();
-    }
-}
-
-template <typename R, bool Dev, int TX, int TY, int BD>
-__global__ __launch_bounds__(TX *TY) void k_fused3(Phys<R> P, const R *__restrict__ f,
-                                                    const R *__restrict__ db_in, R *__restrict__ db_out,
-                                                    R *__restrict__ fout, int lz, uint32_t xg)
-{
-    using D = Fused3Dims<TX, TY>;
-    __shared__ R mring[4][FUSED3_NM][D::MP];
-    __shared__ R dring[3][3][D::DP];
+    __shared__ R aring[3][4][D::MP];  // rho_b - rho_r, da_xyz
+    __shared__ R pring[4][3][D::DP];  // psi_r, psi_b, psi_s
+    __shared__ R sring[2][D::DP];     // rho_s
+    __shared__ R dring[3][3][D::DP];  // new dipole field
     const uint32_t gx = ((uint32_t)P.nx + TX - 1) / TX, gy = ((uint32_t)P.ny + TY - 1) / TY;
     const uint32_t gz = ((uint32_t)P.nz + lz - 1) / (uint32_t)lz;
     const uint32_t tl = xcd_tile(blockIdx.x, gridDim.x, xg);
     if (tl >= gx * gy * gz) return;  // uniform over the block
     const int x0 = (int)(tl % gx) * TX, y0 = (int)((tl / gx) % gy) * TY, z0 = (int)(tl / (gx * gy)) * lz;
     const int zend = min(z0 + lz, P.nz);
-    if (z0 >= zend) return;
-    fused3_tile<R, Dev, TX, TY, BD>(P, f, db_in, db_out, fout, x0, y0, z0, zend, mring, dring);
+    if (z0 >= zend) return;  // uniform over the block
+
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool own = x < P.nx && y < P.ny;
+    // a tile slot past the lattice edge that the wrap still needs as ring
+    const bool own_ring = !own && x <= P.nx + 1 && y <= P.ny + 1;
+    const int qd = (ly + 1) * D::DX + (lx + 1);
+    auto aslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 3) % 3; };
+    auto pslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 4) & 3; };
+    auto sslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 4) & 1; };
+    auto dslot = [&](int zp) __attribute__((always_inline)) { return (zp - z0 + 3) % 3; };
+
+    // the moments of one site into the rings at moments-region point (mx, my)
+    auto put = [&](int zp, int mx, int my, const R (&rho)[3], const R (&da)[3]) __attribute__((always_inline)) {
+        const int q = my * D::MX + mx, sa = aslot(zp);
+        aring[sa][0][q] = rho[1] - rho[0];
+        aring[sa][1][q] = da[0];
+        aring[sa][2][q] = da[1];
+        aring[sa][3][q] = da[2];
+        if (mx >= 1 && mx <= D::MX - 2 && my >= 1 && my <= D::MY - 2) {
+            const int p = (my - 1) * D::DX + (mx - 1), sp = pslot(zp);
+            pring[sp][0][p] = psi_of<Dev>(rho[0], P.emr0[0]);
+            pring[sp][1][p] = psi_of<Dev>(rho[1], P.emr0[1]);
+            pring[sp][2][p] = psi_of<Dev>(rho[2], P.emr0[2]);
+            sring[sslot(zp)][p] = rho[2];
+        }
+    };
+    // M: moments of plane zp; the own site's densities and momenta to (hr, hm)
+    auto fillM = [&](int zp, R (&hr)[3], R (&hm)[3][3]) __attribute__((always_inline)) {
+        const int zw = wrap1(zp, P.nz);
+        if (own) {
+            // opaque copies of the lane coordinates: keeps the compiler from
+            // hoisting the per-direction offsets out of the z loop
+            int xo = x, yo = y;
+            asm volatile("" : "+v"(xo), "+v"(yo));
+            R da0 = R(0), da1 = R(0), da2 = R(0);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hr[c] = R(0);
+                hm[c][0] = hm[c][1] = hm[c][2] = R(0);
+            }
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    R v = pull<R, 3, false>(f, nullptr, P, c, i, xo, yo, zw);
+                    acc_moments<R>(i, v, hr[c], hm[c]);
+                    if (c == 2) acc_da<R, Dev>(P, db_in, i, v, xo, yo, zw, da0, da1, da2);
+                }
+            }
+            const R da[3] = {da0, da1, da2};
+            put(zp, lx + 2, ly + 2, hr, da);
+        } else if (own_ring) {
+            int xo = wrap1(x, P.nx), yo = wrap1(y, P.ny);
+            asm volatile("" : "+v"(xo), "+v"(yo));
+            R rho[3], da[3];
+            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
+            put(zp, lx + 2, ly + 2, rho, da);
+        }
+#pragma unroll
+        for (int r = 0; r < D::NHT; r++) {
+            const int h = tid + r * D::T;
+            if (h >= D::NH) break;
+            int mx, my;
+            D::ring_site(h, mx, my);
+            const int gxs = x0 - 2 + mx, gys = y0 - 2 + my;
+            if (gxs > P.nx + 1 || gys > P.ny + 1) continue;  // beyond a partial tile's needs
+            int xo = wrap1(gxs, P.nx), yo = wrap1(gys, P.ny);
+            asm volatile("" : "+v"(xo), "+v"(yo));
+            R rho[3], da[3];
+            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
+            put(zp, mx, my, rho, da);
+        }
+    };
+    // D: dipole field of plane zp on the tile + 1 site; own sites of planes
+    // z0..zend-1 also go to db_out
+    auto fillD = [&](int zp) __attribute__((always_inline)) {
+        const int dsl = dslot(zp), ss = sslot(zp);
+        const int sa0 = aslot(zp), sam = aslot(zp - 1), sap = aslot(zp + 1);
+        for (int p = tid; p < D::DP; p += D::T) {
+            const int dx = p % D::DX, dy = p / D::DX;
+            const int gxs = x0 - 1 + dx, gys = y0 - 1 + dy;
+            if (gxs > P.nx || gys > P.ny) continue;
+            const int q = (dy + 1) * D::MX + (dx + 1);
+            R out[3];
+            dipole_core<R, Dev>(
+                P, aring[sa0][1][q], aring[sa0][2][q], aring[sa0][3][q], sring[ss][p],
+                [&](int i) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sa0 : (CZ[i] > 0 ? sap : sam);
+                    return aring[sl][0][q + CY[i] * D::MX + CX[i]];
+                },
+                [&](int i, int a) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sa0 : (CZ[i] > 0 ? sap : sam);
+                    return aring[sl][1 + a][q + CY[i] * D::MX + CX[i]];
+                },
+                out);
+            dring[dsl][0][p] = out[0];
+            dring[dsl][1][p] = out[1];
+            dring[dsl][2][p] = out[2];
+            if (dx >= 1 && dx <= TX && dy >= 1 && dy <= TY && gxs < P.nx && gys < P.ny && zp >= z0 && zp < zend) {
+                uint32_t sg = sidx(gxs, gys, zp, P.nx, P.ny);
+                st_plane(db_out, sg, out[0]);
+                st_plane(db_out + (size_t)P.ns, sg, out[1]);
+                st_plane(db_out + (size_t)2 * P.ns, sg, out[2]);
+            }
+        }
+    };
+
+    // densities and momenta of the own site at planes t-2, t-1, t
+    R r0[3], m0[3][3], r1[3], m1[3][3], r2[3], m2[3][3];
+    // iteration t: M of plane t, D of plane t-1, C of plane t-2; the first
+    // iterations fill the rings (M from z0-2, D from z0-1)
+    for (int t = z0 - 2; t < zend + 2; t++) {
+        fillM(t, r2, m2);
+        __syncthreads();
+        if (t > z0 - 1) fillD(t - 1);
+        __syncthreads();
+        const int k = t - 2;
+        if (own && k >= z0) {
+            const int sp0 = pslot(k), spm = pslot(k - 1), spp = pslot(k + 1);
+            const int sd0 = dslot(k), sdm = dslot(k - 1), sdp = dslot(k + 1);
+            const R dbx[3] = {dring[sd0][0][qd], dring[sd0][1][qd], dring[sd0][2][qd]};
+            const R ps_own[3] = {pring[sp0][0][qd], pring[sp0][1][qd], pring[sp0][2][qd]};
+            R F[3][3];
+            sc_forces3<R, Dev>(
+                P, dbx, ps_own,
+                [&](int i, int c) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sp0 : (CZ[i] > 0 ? spp : spm);
+                    return pring[sl][c][qd + CY[i] * D::DX + CX[i]];
+                },
+                [&](int i, int a) __attribute__((always_inline)) {
+                    int sl = CZ[i] == 0 ? sd0 : (CZ[i] > 0 ? sdp : sdm);
+                    return dring[sl][a][qd + CY[i] * D::DX + CX[i]];
+                },
+                F);
+            int xo = x, yo = y;
+            asm volatile("" : "+v"(xo), "+v"(yo));
+            bgk_relax<R, Dev, 3, BD>(
+                P,
+                [&](int c, int i) __attribute__((always_inline)) {
+                    return pull<R, 3, false>(f, nullptr, P, c, i, xo, yo, k);
+                },
+                r0, m0, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            r0[c] = r1[c];
+            r1[c] = r2[c];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                m0[c][a] = m1[c][a];
+                m1[c][a] = m2[c][a];
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------------------------

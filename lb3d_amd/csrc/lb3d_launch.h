@@ -346,12 +346,11 @@ class LauncherT : public Launcher {
         else
             k_fused2<R, Dev, false, TX, TY, SM><<<grid, TX * TY, 0, s>>>(P, f, nullptr, nullptr, fo, lz, xg);
     }
-    // LDS of k_fused3: the moments ring and the dipole ring
+    // LDS of k_fused3: its rings of moments, psi and dipoles
     template <int TX, int TY>
     static constexpr bool fits3()
     {
-        using D3 = Fused3Dims<TX, TY>;
-        return (4 * FUSED3_NM * D3::MP + 9 * D3::DP) * sizeof(R) <= 160 * 1024;
+        return Fused3Dims<TX, TY>::template lds_bytes<R>() <= 160 * 1024;
     }
     template <int TX, int TY>
     bool fused3_t(const DevBufs &b, int lz, hipStream_t s, bool launch)
@@ -383,6 +382,7 @@ class LauncherT : public Launcher {
             if (P.bdist != 0 && P.bdist != 2) return false;
             if (tx == 32 && ty == 4) return fused3_t<32, 4>(b, lz, s, launch);
             if (tx == 32 && ty == 8) return fused3_t<32, 8>(b, lz, s, launch);
+            if (tx == 16 && ty == 16) return fused3_t<16, 16>(b, lz, s, launch);
             if (tx == 64 && ty == 4) return fused3_t<64, 4>(b, lz, s, launch);
             if (tx == 64 && ty == 8) return fused3_t<64, 8>(b, lz, s, launch);
             if (tx == 32 && ty == 16) return fused3_t<32, 16>(b, lz, s, launch);

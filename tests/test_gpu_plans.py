@@ -115,7 +115,7 @@ def test_autotune_keeps_state_and_picks_a_plan(ncomp):
     assert_pops_close(o, g, what=f"after autotune ncomp={ncomp}")
 
 
-TERN_SHAPES = [(32, 4), (32, 8), (64, 4), (64, 8), (32, 16)]
+TERN_SHAPES = [(32, 4), (32, 8), (16, 16), (64, 4), (64, 8), (32, 16)]
 TERN_KW = dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003, amp=(0.02, 0.02, 0.01))
 
 
@@ -133,7 +133,7 @@ def fused3_shapes(precision):
 
 
 def test_ternary_one_pass_shapes_exist():
-    assert len(fused3_shapes(LB3D_FP32_DEV)) == 5
+    assert len(fused3_shapes(LB3D_FP32_DEV)) == 6
     assert (32, 4) in fused3_shapes(LB3D_FP64)
     g = Lattice(8, 8, 8, ncomp=3)
     with pytest.raises(LB3DError):
