@@ -98,8 +98,10 @@ int lb3d_gpu_destroy(lb3d_ctx *ctx);
 /* Copy the host lattice to the device.  whole_N is the reference's AoS array
  * of lbe_site records (code/lb3d_lattice.F90:35-83), Fortran order with x
  * fastest, extent [1-halo, n+halo] in each dimension; site_doubles is 49
- * (binary) or 79 (ternary).  Populations and (ternary) db of the interior are
- * read; the halo, rho, u, da and rock_state are ignored (rock must be 0).
+ * (binary) or 79 (ternary).  Populations, (ternary) db and rock_state of the
+ * interior are read; the halo, rho, u and da are ignored.  A site with
+ * rock_state != 0 is a wall (code/lb3d_bc_bounce.F90:53-90): periodic binary
+ * lattices only, which then run the two-pass plan.
  * Sets the step counter nt = nt0 (tstart). */
 int lb3d_gpu_upload_sites(lb3d_ctx *ctx, const double *whole_N, int32_t halo_extent,
                           int32_t site_doubles, int32_t nt0);
