@@ -548,6 +548,109 @@ void or_bc_init(or_state *o)
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * Zou-He flux boundary, lbe_invade_zouandheflux (code/lb3d_invasion.F90:
+ * 2143-2457), ux = uy = 0: on the post-advection populations of one component
+ * at a site of the z = 1 plane (low) or the z = nz plane, the unknown
+ * populations entering the lattice are set so that the z flux is uz (itype
+ * < 0: uz is a mass flow, divided by the boundary density).  F = n g in the
+ * reference's 1-based direction numbering.
+ */
+#define ZF(k) F[(k) - 1]
+static void zou_he_component(double *n, double uz, int itype, int low)
+{
+    double F[19], rho;
+    const double ux = 0.0, uy = 0.0;
+    for (int s = 0; s < 19; s++) F[s] = n[s] * (double)gdeg[s];
+    if (low) { /* :2179-2214 */
+        if (itype < 0) {
+            rho = ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                  2.0 * (ZF(6) + ZF(10) + ZF(14) + ZF(16) + ZF(18)) + uz;
+            uz = rho > 0.0 ? uz / rho : 0.0;
+        } else {
+            rho = (ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                   2.0 * (ZF(6) + ZF(10) + ZF(14) + ZF(16) + ZF(18))) /
+                  (1.0 - uz);
+        }
+        ZF(5) = ZF(6) + uz * rho / 3.0;
+        if (itype == 0) {
+            ZF(15) = ZF(18) - (ZF(3) - ZF(4)) / 4.0 + rho * uz / 6.0;
+            ZF(17) = ZF(16) + (ZF(3) - ZF(4)) / 4.0 + rho * uz / 6.0;
+            ZF(9) = ZF(14) - (ZF(1) - ZF(2)) / 4.0 + rho * uz / 6.0;
+            ZF(13) = ZF(10) + (ZF(1) - ZF(2)) / 4.0 + rho * uz / 6.0;
+        } else if (itype == 1 || itype == -1) {
+            ZF(15) = ZF(18) - (ZF(7) + ZF(11) + ZF(3) - ZF(8) - ZF(12) - ZF(4)) / 2.0 + uz * rho / 6.0 +
+                     uy * rho / 2.0;
+            ZF(17) = ZF(16) - (ZF(8) + ZF(12) + ZF(4) - ZF(7) - ZF(11) - ZF(3)) / 2.0 + uz * rho / 6.0 -
+                     uy * rho / 2.0;
+            ZF(9) = ZF(14) - (ZF(7) + ZF(8) + ZF(1) - ZF(11) - ZF(12) - ZF(2)) / 2.0 + uz * rho / 6.0 +
+                    ux * rho / 2.0;
+            ZF(13) = ZF(10) - (ZF(11) + ZF(12) + ZF(2) - ZF(7) - ZF(8) - ZF(1)) / 2.0 + uz * rho / 6.0 -
+                     ux * rho / 2.0;
+        }
+    } else { /* :2318-2352 */
+        if (itype < 0) {
+            rho = ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                  2.0 * (ZF(5) + ZF(9) + ZF(13) + ZF(15) + ZF(17)) - uz;
+            uz = rho > 0.0 ? uz / rho : 0.0;
+        } else {
+            rho = (ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                   2.0 * (ZF(5) + ZF(9) + ZF(13) + ZF(15) + ZF(17))) /
+                  (uz + 1.0);
+        }
+        ZF(6) = ZF(5) - uz * rho / 3.0;
+        if (itype == 0) {
+            ZF(18) = ZF(15) + (ZF(3) - ZF(4)) / 4.0 - rho * uz / 6.0;
+            ZF(16) = ZF(17) - (ZF(3) - ZF(4)) / 4.0 - rho * uz / 6.0;
+            ZF(14) = ZF(9) + (ZF(1) - ZF(2)) / 4.0 - rho * uz / 6.0;
+            ZF(10) = ZF(13) - (ZF(1) - ZF(2)) / 4.0 - rho * uz / 6.0;
+        } else if (itype == 1 || itype == -1) {
+            ZF(18) = ZF(15) + (ZF(7) + ZF(11) + ZF(3) - ZF(8) - ZF(12) - ZF(4)) / 2.0 - uz * rho / 6.0 -
+                     uy * rho / 2.0;
+            ZF(16) = ZF(17) + (ZF(8) + ZF(12) + ZF(4) - ZF(7) - ZF(11) - ZF(3)) / 2.0 - uz * rho / 6.0 +
+                     uy * rho / 2.0;
+            ZF(14) = ZF(9) + (ZF(7) + ZF(8) + ZF(1) - ZF(11) - ZF(12) - ZF(2)) / 2.0 - uz * rho / 6.0 -
+                     ux * rho / 2.0;
+            ZF(10) = ZF(13) + (ZF(11) + ZF(12) + ZF(2) - ZF(7) - ZF(8) - ZF(1)) / 2.0 - uz * rho / 6.0 +
+                     ux * rho / 2.0;
+        }
+    }
+    for (int s = 0; s < 19; s++) n[s] = F[s] / (double)gdeg[s];
+}
+#undef ZF
+
+void or_zou_he_site(double *n, double uz, int itype, int low) { zou_he_component(n, uz, itype, low); }
+
+/* The boundary on both z planes of the fluid sites: surfactant first, with
+ * the advected dipole of the site recomputed from the new populations and the
+ * neighbours' dipole field, then red and blue.  rho and u keep the values of
+ * the advection. */
+static void zou_he_flux(or_state *o)
+{
+    const int nx = o->p.nx, ny = o->p.ny, C = o->p.ncomp, ro = off_rock(o);
+    const int itype = o->p.inv_type;
+    const int zs[2] = {1, o->p.nz};
+    for (int b = 0; b < 2; b++) { /* nz = 1: both rules on the one plane, low first */
+        const int z = zs[b], low = b == 0;
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++) {
+                double *S = site(o, x, y, z);
+                if (S[ro] != 0.0) continue;
+                if (C == 3) {
+                    zou_he_component(S + OFF_N(2), o->p.inv_flux[2], itype, low);
+                    S[OFF_DA] = S[OFF_DA + 1] = S[OFF_DA + 2] = 0.0;
+                    for (int s = 0; s < 19; s++) {
+                        const double *Sb = csite(o, x - cx[s], y - cy[s], z - cz[s]);
+                        for (int k = 0; k < 3; k++)
+                            S[OFF_DA + k] = S[OFF_DA + k] + (S[OFF_N(2) + s] * (double)gdeg[s]) * Sb[OFF_DB + k];
+                    }
+                }
+                zou_he_component(S + OFF_N(0), o->p.inv_flux[0], itype, low);
+                zou_he_component(S + OFF_N(1), o->p.inv_flux[1], itype, low);
+            }
+    }
+}
+
 void or_bc_apply(or_state *o, int stage)
 {
     if (stage == 6) { /* :106-122 */
@@ -559,7 +662,8 @@ void or_bc_apply(or_state *o, int stage)
         } else {
             halo_exchange_local(o);
         }
-    } else if (stage == 8) { /* :124-146; invasion is a no-op for inv_fluid 5/-1 */
+    } else if (stage == 8) { /* :124-146: invasion (lbe_invade) for inv_fluid 12, then the halo */
+        if (o->p.inv_fluid == 12 && !o->p.lees_edwards) zou_he_flux(o);
         if (o->p.lees_edwards) {
             le_halo_exchange(o);
             update_moments_loop(o);

@@ -44,6 +44,12 @@ typedef struct or_params {
      * relaxation weights, rock colours of lbe_init_rock_colour */
     double g_wr, g_wb, tau_wr, tau_wb;
     double rock_colour_r, rock_colour_b;
+    /* invasion at stage 8 (code/lb3d_invasion.F90:45-129): inv_fluid 12 is the
+     * Zou-He flux boundary on the z planes (lbe_invade_zouandheflux), any other
+     * value none; inv_type -1, 0 or 1; inv_flux = (pr, pb, pg), the z flux of
+     * red, blue and surfactant */
+    int32_t inv_fluid, inv_type;
+    double inv_flux[3];
 } or_params;
 
 typedef struct or_state or_state;
@@ -64,6 +70,9 @@ void      or_set_nt(or_state *o, int nt);
 void or_bc_init(or_state *o);
 /* the reference's call surface, one stage at a time */
 void or_bc_apply(or_state *o, int stage);    /* stages 6 and 8 */
+/* the Zou-He flux rule on the 19 populations of one component at one site of
+ * the z = 1 plane (low = 1) or the z = nz plane (low = 0) */
+void or_zou_he_site(double *n, double uz, int itype, int low);
 void or_advection(or_state *o);              /* stage 7 */
 void or_collision(or_state *o);              /* stage 9 */
 /* one timestep nt -> nt+1 (code/lb3d_time_loop.F90:58-92) and n of them */

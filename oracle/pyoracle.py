@@ -34,6 +34,8 @@ class OrParams(ctypes.Structure):
         ("g_wr", ctypes.c_double), ("g_wb", ctypes.c_double),
         ("tau_wr", ctypes.c_double), ("tau_wb", ctypes.c_double),
         ("rock_colour_r", ctypes.c_double), ("rock_colour_b", ctypes.c_double),
+        ("inv_fluid", ctypes.c_int32), ("inv_type", ctypes.c_int32),
+        ("inv_flux", ctypes.c_double * 3),
     ]
 
 
@@ -88,6 +90,7 @@ def lib():
             getattr(L, name).argtypes = [vp, dp]
         L.or_boltz_dist.argtypes = [ip] + [ctypes.c_double] * 6 + [dp]
         L.or_boltz_dist_dd.argtypes = [ip] + [ctypes.c_double] * 9 + [dp]
+        L.or_zou_he_site.argtypes = [dp, ctypes.c_double, ip, ip]
         L.or_sc_force.argtypes = [vp, ip, ip, ip, ctypes.POINTER(ctypes.c_double * 9)]
         L.or_init_noise.argtypes = [vp, ctypes.c_uint64, dp, dp, ip]
         L.or_hash_uniform.restype = ctypes.c_double
@@ -117,6 +120,7 @@ DEFAULTS = dict(
     g_br=0.08, g_bs=-0.006, g_ss=-0.003, shear_u=0.0, shear_omega=0.0, shear_sum=0.0,
     g_accn=(0.0, 0.0, 0.0), g_accn_min=(0, 0, 0), g_accn_max=None,
     g_wr=0.0, g_wb=0.0, tau_wr=1.0, tau_wb=1.0, rock_colour_r=0.0, rock_colour_b=0.0,
+    inv_fluid=-1, inv_type=0, inv_flux=(0.0, 0.0, 0.0),
 )
 
 
@@ -125,8 +129,9 @@ def make_params(nx, ny, nz, **kw):
     d.update(kw)
     p = OrParams()
     p.nx, p.ny, p.nz = nx, ny, nz
-    for k in ("ncomp", "bdist", "lees_edwards", "restore"):
+    for k in ("ncomp", "bdist", "lees_edwards", "restore", "inv_fluid", "inv_type"):
         setattr(p, k, int(d[k]))
+    p.inv_flux[:] = [float(v) for v in d["inv_flux"]]
     for k in ("tau_d", "beta", "d_0", "g_br", "g_bs", "g_ss", "shear_u", "shear_omega",
               "shear_sum", "g_wr", "g_wb", "tau_wr", "tau_wb", "rock_colour_r", "rock_colour_b"):
         setattr(p, k, float(d[k]))

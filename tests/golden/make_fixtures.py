@@ -1,7 +1,7 @@
 """Write tests/golden/reference_goldens.npz from the reference's own golden tests.
 
 Reads, as data only, the checkpoint inputs and the float32 XDR dumps that the
-reference's regression tests hold (test/{gyroid,leesEdwards,bodyForce,wettingWall}/,
+reference's regression tests hold (test/{gyroid,leesEdwards,bodyForce,wettingWall,zouHe}/,
 run by testLB3D.sh), and stores them unchanged in one compressed npz so that
 tests run where the reference tree is absent.  Formats:
   checkpoint_*_p000000.xdr  big-endian FP64 records, x outer .. z inner,
@@ -22,6 +22,7 @@ TESTS = {
     "leesEdwards": (39, ["od", "wd", "vel", "arr"]),
     "bodyForce": (64, ["od", "wd", "sur", "dir", "vel", "arr"]),
     "wettingWall": (39, ["od", "wd", "vel", "arr"]),
+    "zouHe": (64, ["od", "wd", "sur", "dir", "vel", "arr"]),
 }
 
 

@@ -28,6 +28,11 @@ CASES = {
     # colours the rock red, g_wr = -0.1)
     "wettingWall": dict(ncomp=2, lees_edwards=0, g_br=0.12, g_wr=-0.1, rock_colour_r=1.0, beta=1.0,
                         obs=["od", "wd", "vel", "arr"]),
+    # ternary with the Zou-He flux boundary on the z planes (boundary_cond = -1,
+    # inv_fluid = 12, inv_type = -1, red mass flow pr = 1e-6)
+    "zouHe": dict(ncomp=3, lees_edwards=0, g_br=0.08, g_bs=-0.006, g_ss=0.003, beta=1.0, d_0=1.0,
+                  tau_d=1.0, inv_fluid=12, inv_type=-1, inv_flux=(1e-6, 0.0, 0.0),
+                  obs=["od", "wd", "sur", "dir", "vel", "arr"]),
 }
 
 

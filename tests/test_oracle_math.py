@@ -323,3 +323,39 @@ def test_hudong_iterated_exponential():
         errs.append((abs(b.sum() - 1.0), np.abs(C3.T @ b - u).max()))
     assert all(errs[k + 1][0] < errs[k][0] for k in range(3))
     assert errs[-1][0] < 3e-4 and errs[-1][1] < 2e-4
+
+
+def zou_he(n, uz, itype, low):
+    buf = (ctypes.c_double * 19)(*[float(v) for v in n])
+    lib().or_zou_he_site(buf, float(uz), int(itype), int(low))
+    return np.array(buf[:])
+
+
+@pytest.mark.parametrize("itype", [-1, 0, 1])
+@pytest.mark.parametrize("low", [1, 0])
+def test_zou_he_flux_rule_moments(itype, low):
+    """lbe_invade_zouandheflux (code/lb3d_invasion.F90:2143-2457) on one site:
+    only the populations entering the lattice change; with K, O, I the sums
+    of g n over c_z = 0, leaving and entering, mass and z momentum fix
+    rho = K + 2 O + m (mass flow m, itype < 0) or (K + 2 O) / (1 - uz), and
+    the z momentum rho uz (= m for itype < 0), in +z on both planes.  The
+    corrected diagonals (itype +-1) leave no momentum along x and y."""
+    rng = np.random.default_rng(7 + itype + 3 * low)
+    n = 0.02 + 0.01 * rng.random(19)
+    cz = C3[:, 2]
+    enter = cz > 0 if low else cz < 0
+    flux = 0.013
+    n2 = zou_he(n, flux, itype, low)
+    np.testing.assert_array_equal(n2[~enter], n[~enter])
+    F = G * n
+    K, O = F[cz == 0].sum(), F[~enter & (cz != 0)].sum()
+    if itype < 0:
+        rho, mz = K + 2 * O + (flux if low else -flux), flux
+    else:
+        rho = (K + 2 * O) / (1 - flux if low else 1 + flux)
+        mz = rho * flux
+    F2 = G * n2
+    np.testing.assert_allclose(F2.sum(), rho, rtol=1e-14)
+    np.testing.assert_allclose(F2 @ cz, mz, rtol=1e-12)
+    if itype != 0:
+        np.testing.assert_allclose(F2 @ C3[:, :2], [0.0, 0.0], atol=1e-15)
