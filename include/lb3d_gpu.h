@@ -79,6 +79,14 @@ typedef struct lb3d_params {
      * the colours rock sites carry (lbe_init_rock_colour, code/lb3d_ic_rock.F90:146-189) */
     double g_wr, g_wb, tau_wr, tau_wb;
     double rock_colour_r, rock_colour_b;
+    /* invasion after the advection (lbe_invade, code/lb3d_invasion.F90:45-129,
+     * with boundary_cond != 0): inv_fluid 0 none, 12 the Zou-He flux boundary
+     * on the z planes (lbe_invade_zouandheflux, :2143-2457; periodic lattices
+     * without walls); inv_type -1 (the flux is a mass flow), 0 or 1 (with the
+     * corrected diagonals); inv_flux = (pr, pb, pg), the z flux of red, blue
+     * and surfactant */
+    int32_t inv_fluid, inv_type;
+    double inv_flux[3];
 } lb3d_params;
 
 typedef struct lb3d_ctx lb3d_ctx;

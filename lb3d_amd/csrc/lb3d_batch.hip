@@ -124,6 +124,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
         (void)probe;
         if (q.ncomp != 2 && q.ncomp != 3) return set_error(LB3D_EINVAL, "ncomp must be 2 or 3");
         if (q.bdist != 0 && q.bdist != 2) return set_error(LB3D_EINVAL, "bdist must be 0 or 2");
+        if (q.inv_fluid != 0) return set_error(LB3D_EINVAL, "the batch runs lattices without invasion");
         if (q.precision != LB3D_FP64 && q.precision != LB3D_FP32_DEV)
             return set_error(LB3D_EINVAL, "precision must be LB3D_FP64 or LB3D_FP32_DEV");
         if (q.nx < 2 || q.ny < 1 || q.nz < 1) return set_error(LB3D_EINVAL, "lattice extent must be nx>=2, ny>=1, nz>=1");

@@ -92,6 +92,12 @@ static int check_params(const lb3d_params *p)
         if (!(p->tau[c] > 0.0) || !(p->amass[c] > 0.0))
             return fail(LB3D_EINVAL, "tau and amass must be positive");
     if (p->ncomp == 3 && !(p->tau_d > 0.0)) return fail(LB3D_EINVAL, "tau_d must be positive");
+    if (p->inv_fluid != 0 && p->inv_fluid != 12)
+        return fail(LB3D_EINVAL, "inv_fluid must be 0 (no invasion) or 12 (the Zou-He flux boundary)");
+    if (p->inv_fluid == 12 && p->boundary == LB3D_BC_LEES_EDWARDS)
+        return fail(LB3D_EINVAL, "the Zou-He boundary needs a periodic lattice");
+    if (p->inv_fluid == 12 && (p->inv_type < -1 || p->inv_type > 1))
+        return fail(LB3D_EINVAL, "inv_type must be -1, 0 or 1 (code/lb3d_invasion.F90:2136-2140)");
     double sites = (double)p->nx * (double)p->ny * (double)p->nz;
     if (sites * 19.0 * p->ncomp >= 4294967296.0)
         return fail(LB3D_EINVAL, "lattice too large for 32-bit population indexing");
@@ -354,6 +360,8 @@ int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, i
     if (rc) return rc;
     if (nrock && (c->p.ncomp != 2 || c->le))
         return fail(LB3D_EINVAL, "rock sites (walls) are supported on periodic binary lattices");
+    if (nrock && c->p.inv_fluid == 12)
+        return fail(LB3D_EINVAL, "rock sites (walls) and the Zou-He boundary are not supported together");
     if ((rc = set_walls(c, nrock ? rock.data() : nullptr))) return rc;
     c->b.cur = 0;
     HIP_TRY(hipMemcpyAsync(c->b.f[0], fb.data(), c->f_bytes, hipMemcpyHostToDevice, c->stream));

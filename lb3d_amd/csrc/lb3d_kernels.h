@@ -169,7 +169,90 @@ __device__ __forceinline__ void acc_da(const Phys<R> &P, const R *__restrict__ d
     da2 = da2 + ng * ld_plane(plane_of(db, 2u, P.ns), q);
 }
 
-template <typename R, bool Dev, int C, bool LE, bool Rock = false>
+// Zou-He flux rule, lbe_invade_zouandheflux (code/lb3d_invasion.F90:2143-2457)
+// with ux = uy = 0, on the streamed populations n (stored form) of one
+// component at a site of the z = 0 plane (low) or the z = nz-1 plane: the
+// populations entering the lattice are rebuilt so that the z flux is uz
+// (itype < 0: uz is a mass flow, divided by the boundary density).  F = g n
+// in the reference's 1-based direction numbering.  In the deviation form
+// F = g (n - w rho0): the rest state satisfies every relation, so they hold
+// for the deviations as written, with the boundary density rho0 + their sum.
+#define ZF(k) F[(k) - 1]
+template <typename R, bool Dev>
+__device__ __forceinline__ void zou_he_site(R (&n)[19], R uz, int itype, bool low, R rho0)
+{
+    R F[19];
+#pragma unroll
+    for (int s = 0; s < 19; s++) F[s] = n[s] * R(GD[s]);
+    const R ux = R(0), uy = R(0);
+    R rho;
+    if (low) {  // :2179-2214
+        R sum = ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                R(2.0) * (ZF(6) + ZF(10) + ZF(14) + ZF(16) + ZF(18));
+        if constexpr (Dev) sum = rho0 + sum;
+        if (itype < 0) {
+            rho = sum + uz;
+            uz = rho > R(0) ? uz / rho : R(0);
+        } else {
+            rho = sum / (R(1.0) - uz);
+        }
+        ZF(5) = ZF(6) + uz * rho / R(3.0);
+        if (itype == 0) {
+            ZF(15) = ZF(18) - (ZF(3) - ZF(4)) / R(4.0) + rho * uz / R(6.0);
+            ZF(17) = ZF(16) + (ZF(3) - ZF(4)) / R(4.0) + rho * uz / R(6.0);
+            ZF(9) = ZF(14) - (ZF(1) - ZF(2)) / R(4.0) + rho * uz / R(6.0);
+            ZF(13) = ZF(10) + (ZF(1) - ZF(2)) / R(4.0) + rho * uz / R(6.0);
+        } else {
+            ZF(15) = ZF(18) - (ZF(7) + ZF(11) + ZF(3) - ZF(8) - ZF(12) - ZF(4)) / R(2.0) + uz * rho / R(6.0) +
+                     uy * rho / R(2.0);
+            ZF(17) = ZF(16) - (ZF(8) + ZF(12) + ZF(4) - ZF(7) - ZF(11) - ZF(3)) / R(2.0) + uz * rho / R(6.0) -
+                     uy * rho / R(2.0);
+            ZF(9) = ZF(14) - (ZF(7) + ZF(8) + ZF(1) - ZF(11) - ZF(12) - ZF(2)) / R(2.0) + uz * rho / R(6.0) +
+                    ux * rho / R(2.0);
+            ZF(13) = ZF(10) - (ZF(11) + ZF(12) + ZF(2) - ZF(7) - ZF(8) - ZF(1)) / R(2.0) + uz * rho / R(6.0) -
+                     ux * rho / R(2.0);
+        }
+    } else {  // :2318-2352
+        R sum = ZF(19) + ZF(1) + ZF(2) + ZF(3) + ZF(4) + ZF(7) + ZF(11) + ZF(12) + ZF(8) +
+                R(2.0) * (ZF(5) + ZF(9) + ZF(13) + ZF(15) + ZF(17));
+        if constexpr (Dev) sum = rho0 + sum;
+        if (itype < 0) {
+            rho = sum - uz;
+            uz = rho > R(0) ? uz / rho : R(0);
+        } else {
+            rho = sum / (uz + R(1.0));
+        }
+        ZF(6) = ZF(5) - uz * rho / R(3.0);
+        if (itype == 0) {
+            ZF(18) = ZF(15) + (ZF(3) - ZF(4)) / R(4.0) - rho * uz / R(6.0);
+            ZF(16) = ZF(17) - (ZF(3) - ZF(4)) / R(4.0) - rho * uz / R(6.0);
+            ZF(14) = ZF(9) + (ZF(1) - ZF(2)) / R(4.0) - rho * uz / R(6.0);
+            ZF(10) = ZF(13) - (ZF(1) - ZF(2)) / R(4.0) - rho * uz / R(6.0);
+        } else {
+            ZF(18) = ZF(15) + (ZF(7) + ZF(11) + ZF(3) - ZF(8) - ZF(12) - ZF(4)) / R(2.0) - uz * rho / R(6.0) -
+                     uy * rho / R(2.0);
+            ZF(16) = ZF(17) + (ZF(8) + ZF(12) + ZF(4) - ZF(7) - ZF(11) - ZF(3)) / R(2.0) - uz * rho / R(6.0) +
+                     uy * rho / R(2.0);
+            ZF(14) = ZF(9) + (ZF(7) + ZF(8) + ZF(1) - ZF(11) - ZF(12) - ZF(2)) / R(2.0) - uz * rho / R(6.0) -
+                     ux * rho / R(2.0);
+            ZF(10) = ZF(13) + (ZF(11) + ZF(12) + ZF(2) - ZF(7) - ZF(8) - ZF(1)) / R(2.0) - uz * rho / R(6.0) +
+                     ux * rho / R(2.0);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 19; s++) n[s] = F[s] / R(GD[s]);
+}
+#undef ZF
+
+// The rule on both planes a site may sit on (nz = 1: the low one first).
+template <typename R, bool Dev>
+__device__ __forceinline__ void zou_he_planes(const Phys<R> &P, R (&n)[19], int c, int z)
+{
+    if (z == 0) zou_he_site<R, Dev>(n, P.zh_flux[c], P.zh_type, true, P.rho0[c]);
+    if (z == P.nz - 1) zou_he_site<R, Dev>(n, P.zh_flux[c], P.zh_type, false, P.rho0[c]);
+}
+
+template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false>
 __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, int x, int y, int z, R (&rho)[C],
@@ -179,6 +262,29 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
 #pragma unroll
     for (int c = 0; c < C; c++) rho[c] = R(0);
     R da0 = R(0), da1 = R(0), da2 = R(0);
+    if constexpr (ZH && C == 3) {
+        if (z == 0 || z == P.nz - 1) {
+            // Zou-He planes: rho of the streamed populations, the advected
+            // dipole from the rebuilt surfactant populations
+            // (code/lb3d_invasion.F90:2216-2228)
+            R ns[19];
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    R v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+                    rho[c] = rho[c] + v * R(GD[i]);
+                    if (c == 2) ns[i] = v;
+                }
+            zou_he_planes<R, Dev>(P, ns, 2, z);
+#pragma unroll
+            for (int i = 0; i < 19; i++) acc_da<R, Dev>(P, db, i, ns[i], x, y, z, da0, da1, da2);
+            da[0] = da0;
+            da[1] = da1;
+            da[2] = da2;
+            return;
+        }
+    }
     bool self_rock = false;
     if constexpr (Rock) self_rock = rock[sidx(x, y, z, P.nx, P.ny)] != 0;
 #pragma unroll
@@ -200,7 +306,7 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
     da[2] = da2;
 }
 
-template <typename R, bool Dev, int C, bool LE, bool Rock = false>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false>
 __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, R *__restrict__ aux,
@@ -209,7 +315,7 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R rho[C], da[3];
-    moments_core<R, Dev, C, LE, Rock>(P, f, face_pc, db, x, y, z, rho, da, rock);
+    moments_core<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, x, y, z, rho, da, rock);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
@@ -222,7 +328,7 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
     }
 }
 
-template <typename R, bool Dev, int C, bool LE, bool Rock = false>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false>
 __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ db, R *__restrict__ aux,
@@ -230,7 +336,7 @@ __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict_
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    moments_site<R, Dev, C, LE, Rock>(P, f, face_pc, db, aux, s, rock);
+    moments_site<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, aux, s, rock);
 }
 
 // --------------------------------------------------------------------------
@@ -608,7 +714,7 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
 // --------------------------------------------------------------------------
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
-template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false>
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false>
 __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ aux,
@@ -641,10 +747,33 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
             for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
         sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
     }
+    if constexpr (ZH) {
+        if (z == 0 || z == P.nz - 1) {
+            // Zou-He planes: the rebuilt populations relax towards the
+            // equilibrium of the streamed state's moments, which the reference
+            // keeps from the advection (code/lb3d_collision.F90:249-289, 560-587)
+            R rho[C], mom[C][3];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                rho[c] = R(0);
+                mom[c][0] = mom[c][1] = mom[c][2] = R(0);
+            }
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+#pragma unroll
+                for (int c = 0; c < C; c++) acc_moments<R>(i, n[c][i], rho[c], mom[c]);
+#pragma unroll
+            for (int c = 0; c < C; c++) zou_he_planes<R, Dev>(P, n[c], c, z);
+            bgk_relax<R, Dev, C, BD>(
+                P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, rho, mom, F, x, y, z, s,
+                fout);
+            return;
+        }
+    }
     bgk_site<R, Dev, C, BD>(P, n, F, x, y, z, s, fout);
 }
 
-template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false>
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false>
 __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ aux,
@@ -654,7 +783,7 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    collide_site<R, Dev, C, LE, BD, Rock>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
+    collide_site<R, Dev, C, LE, BD, Rock, ZH>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
 }
 
 // --------------------------------------------------------------------------

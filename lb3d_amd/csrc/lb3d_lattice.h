@@ -46,6 +46,9 @@ struct Phys {
     R rho0[3], wrho0_move[3], wrho0_rest[3], psi0[3], emr0[3];
     // walls: g_wr, g_wb, tau_wr/tau_r, tau_wb/tau_b as the force uses them, rock colours
     R g_wr, g_wb, tau_wr, tau_wb, rock_col[2];
+    // Zou-He flux boundary on the z planes: on, inv_type, z flux per component
+    int zh, zh_type;
+    R zh_flux[3];
 };
 
 // Lees-Edwards scalars, updated on the device once per step (FP64).

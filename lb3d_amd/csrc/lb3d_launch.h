@@ -56,6 +56,11 @@ Phys<R> make_phys(const lb3d_params &p)
     P.tau_wb = R(p.tau_wb);
     P.rock_col[0] = R(p.rock_colour_r);
     P.rock_col[1] = R(p.rock_colour_b);
+    P.zh = p.inv_fluid == 12 ? 1 : 0;
+    P.zh_type = p.inv_type;
+    P.zh_flux[0] = R(p.inv_flux[0]);
+    P.zh_flux[1] = R(p.inv_flux[1]);
+    P.zh_flux[2] = R(p.inv_flux[2]);
     return P;
 }
 
@@ -252,7 +257,10 @@ class LauncherT : public Launcher {
     void moments(const DevBufs &b, hipStream_t s) override
     {
         const R *f = (const R *)b.f[b.cur];
-        if (C == 3)
+        if (C == 3 && P.zh)
+            k_moments<R, Dev, 3, false, false, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db,
+                                                                               (R *)b.aux, xg, nullptr);
+        else if (C == 3)
             k_moments<R, Dev, 3, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.db, (R *)b.aux, xg,
                                                                   nullptr);
         else if (le)
@@ -273,7 +281,13 @@ class LauncherT : public Launcher {
     {
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
-        if (C == 3)
+        if (P.zh && C == 3)
+            k_collide<R, Dev, 3, false, BD, false, true><<<tiles(), BLOCK, 0, s>>>(
+                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db, fo, xg, nullptr);
+        else if (P.zh && !le && !b.rock)
+            k_collide<R, Dev, 2, false, BD, false, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
+                                                                                 nullptr, nullptr, fo, xg, nullptr);
+        else if (C == 3)
             k_collide<R, Dev, 3, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
                                                                       (const R *)b.db, fo, xg, nullptr);
         else if (le)
@@ -375,7 +389,7 @@ class LauncherT : public Launcher {
     }
     bool fused_dispatch(const DevBufs &b, int tx, int ty, int lz, int st, hipStream_t s, bool launch)
     {
-        if (walls) return false;  // walls run on the two-pass kernels
+        if (walls || P.zh) return false;  // walls and the Zou-He boundary run on the two-pass kernels
         if (C == 3) {
             // ternary: the reload form only, periodic lattices, nz >= 3 for the +-2 plane wrap
             if (le || lz < 1 || st != 2 || P.nz < 3 || P.nx < 3 || P.ny < 3) return false;

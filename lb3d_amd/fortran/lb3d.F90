@@ -99,6 +99,15 @@ contains
     p%g_accn_max = [g_accn_max_x, g_accn_max_y, g_accn_max]
     p%g_wr = g_wr; p%g_wb = g_wb; p%tau_wr = tau_wr; p%tau_wb = tau_wb
     p%rock_colour_r = rock_colour_r; p%rock_colour_b = rock_colour_b
+    ! invasion after the advection (lbe_invade, code/lb3d_bc.F90:129-133): the
+    ! Zou-He flux boundary (inv_fluid 12) with (uz_r, uz_g, uz_b) = (pr, pg, pb)
+    if (boundary_cond /= 0 .and. inv_fluid >= 0 .and. all(inv_fluid /= [5, 6, 12])) then
+       write(*, '(A,I0)') 'FATAL ERROR: this invasion type is not supported: inv_fluid = ', inv_fluid
+       error stop 1
+    end if
+    p%inv_fluid = merge(12, 0, boundary_cond /= 0 .and. inv_fluid == 12)
+    p%inv_type = inv_type
+    p%inv_flux = [pr, pb, pg]
     ! reference densities of the FP32 deviation form: the mean of the initial state
     p%rho0 = 0.0_c_double
     if (restored) then

@@ -33,6 +33,8 @@ module lb3d_gpu_module
      real(c_double) :: rho0(3)
      real(c_double) :: g_wr, g_wb, tau_wr, tau_wb
      real(c_double) :: rock_colour_r, rock_colour_b
+     integer(c_int32_t) :: inv_fluid, inv_type
+     real(c_double) :: inv_flux(3)
   end type lb3d_params
 
   !> lb3d_sanity: C [species][axis][xyz] arrays appear here as (xyz, axis, species)

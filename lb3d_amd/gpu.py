@@ -84,6 +84,8 @@ class Params(ctypes.Structure):
         ("g_wr", ctypes.c_double), ("g_wb", ctypes.c_double),
         ("tau_wr", ctypes.c_double), ("tau_wb", ctypes.c_double),
         ("rock_colour_r", ctypes.c_double), ("rock_colour_b", ctypes.c_double),
+        ("inv_fluid", ctypes.c_int32), ("inv_type", ctypes.c_int32),
+        ("inv_flux", ctypes.c_double * 3),
     ]
 
 
@@ -152,7 +154,8 @@ def make_params(nx, ny, nz, ncomp=2, bdist=2, precision=LB3D_FP64, boundary=LB3D
                 d_0=1.0, g_br=0.08, g_bs=-0.006, g_ss=-0.003, shear_u=0.0, shear_omega=0.0,
                 shear_sum=0.0, g_accn=(0.0, 0.0, 0.0), g_accn_min=(0, 0, 0),
                 g_accn_max=(0, 0, 0), rho0=(0.7, 0.7, 0.7), g_wr=0.0, g_wb=0.0, tau_wr=1.0,
-                tau_wb=1.0, rock_colour_r=0.0, rock_colour_b=0.0):
+                tau_wb=1.0, rock_colour_r=0.0, rock_colour_b=0.0, inv_fluid=0, inv_type=0,
+                inv_flux=(0.0, 0.0, 0.0)):
     p = Params()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.ncomp, p.bdist, p.precision, p.boundary, p.restore = ncomp, bdist, precision, boundary, restore
@@ -167,6 +170,8 @@ def make_params(nx, ny, nz, ncomp=2, bdist=2, precision=LB3D_FP64, boundary=LB3D
     p.rho0[:] = list(map(float, rho0))
     p.g_wr, p.g_wb, p.tau_wr, p.tau_wb = float(g_wr), float(g_wb), float(tau_wr), float(tau_wb)
     p.rock_colour_r, p.rock_colour_b = float(rock_colour_r), float(rock_colour_b)
+    p.inv_fluid, p.inv_type = int(inv_fluid), int(inv_type)
+    p.inv_flux[:] = list(map(float, inv_flux))
     return p
 
 

@@ -34,6 +34,9 @@ def test_library_exports_every_declared_symbol():
     (dict(bdist=7), "bdist"),
     (dict(precision=7), "precision"),
     (dict(tau=(0.0, 1.0, 1.0)), "tau"),
+    (dict(inv_fluid=11), "inv_fluid"),
+    (dict(inv_fluid=12, inv_type=2), "inv_type"),
+    (dict(inv_fluid=12, boundary=gpu.LB3D_BC_LEES_EDWARDS), "Zou-He"),
 ])
 def test_invalid_parameters_rejected(kw, msg):
     with pytest.raises(gpu.LB3DError) as e:
@@ -48,7 +51,7 @@ def test_too_large_lattice_rejected():
 
 def test_params_struct_layout():
     # the Fortran bind(C) type and the C struct must agree on this size
-    assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3 + 8 * 6
+    assert ctypes.sizeof(gpu.Params) == 4 * 8 + 8 * (3 + 3 + 3 + 3 + 3 + 3) + 4 * 6 + 8 * 3 + 8 * 6 + 4 * 2 + 8 * 3
 
 
 def test_sanity_struct_layout():

@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 CASES = {"gyroid": (3, ["od", "wd", "sur", "dir", "vel"]),
          "leesEdwards": (2, ["od", "wd", "vel"]),
          "bodyForce": (3, ["od", "wd", "sur", "dir", "vel"]),
-         "wettingWall": (2, ["od", "wd", "vel"])}
+         "wettingWall": (2, ["od", "wd", "vel"]),
+         "zouHe": (3, ["od", "wd", "sur", "dir", "vel"])}
 
 
 def write_checkpoint(dirname, name, gold):

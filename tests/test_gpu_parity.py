@@ -170,7 +170,7 @@ def test_stage_order_is_enforced():
     assert g.nt == 1
 
 
-@pytest.mark.parametrize("name", ["gyroid", "leesEdwards", "bodyForce", "wettingWall"])
+@pytest.mark.parametrize("name", ["gyroid", "leesEdwards", "bodyForce", "wettingWall", "zouHe"])
 def test_reference_golden_tests_on_gpu(name):
     """Restore the reference test's checkpoint on the GPU, run its 10 steps,
     and reproduce its float32 dumps exactly (od, wd, sur, dir, vel, arr)."""
@@ -197,6 +197,33 @@ def test_reference_golden_tests_on_gpu(name):
     for ob in obs:
         mine = chk.dump(ob).astype(np.float32)
         np.testing.assert_array_equal(mine, gold[f"{name}/{ob}"], err_msg=f"{name}/{ob}")
+
+
+@pytest.mark.parametrize("itype", [-1, 0, 1])
+@pytest.mark.parametrize("ncomp,dims", [(3, (9, 7, 6)), (2, (8, 6, 5)), (3, (6, 5, 1))])
+def test_zou_he_flux_boundary(itype, ncomp, dims):
+    """The Zou-He flux boundary on the z planes (code/lb3d_invasion.F90:
+    2143-2457) with fluxes large enough to show in FP64 (the reference's
+    zouHe golden uses 1e-6), against the oracle; nz = 1 applies both rules to
+    one plane.  FP32 deviation storage follows the FP64 form."""
+    kw = dict(ncomp=ncomp, g_br=0.08, inv_fluid=12, inv_type=itype, inv_flux=(0.004, -0.002, 0.003))
+    o, g = pair(*dims, amp=(0.02, 0.02, 0.01), **kw)
+    o.run(12)
+    g.time_loop(12)
+    assert_pops_close(o, g, what=f"Zou-He itype {itype} ncomp {ncomp} {dims}")
+    N = g.download_sites()
+    I = o.interior()
+    if ncomp == 3:
+        np.testing.assert_allclose(N[1:-1, 1:-1, 1:-1, 72:75], I[..., 72:75], rtol=0, atol=1e-13)
+    _, g32 = pair(*dims, amp=(0.02, 0.02, 0.01), precision=LB3D_FP32_DEV, **kw)
+    g32.time_loop(12)
+    a, b = g.populations(), g32.populations()
+    rel = np.abs(a - b).max() / np.abs(a).max()
+    print(f"Zou-He FP32 vs FP64 {rel:.3e}")
+    assert rel < 2e-6
+    # the one-pass plans do not take the boundary
+    with pytest.raises(LB3DError):
+        g.set_plan(3, 32, 4, 4)
 
 
 def test_walls_fp32_close_to_fp64_and_rules():
