@@ -1,0 +1,131 @@
+"""The benchmark configurations at their full sizes, through properties that
+need no oracle run: per-component mass conservation of the FP64 form,
+results that do not depend on the execution plan (bitwise), the FP32
+deviation form against FP64, the 48-lattice ensemble against its lattices run
+one by one, and a ternary lattice at the 32-bit population-indexing limit."""
+import numpy as np
+import pytest
+
+from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64, Batch, LB3DError, Lattice
+
+pytestmark = pytest.mark.gpu
+
+BINARY = dict(ncomp=2, g_br=0.08)
+TERNARY = dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0)
+
+
+def lattice(dims, precision, seed=1, **kw):
+    g = Lattice(*dims, precision=precision, **kw)
+    g.init_noise(seed, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+    g.bc_init()
+    return g
+
+
+def mass_drift(g, steps):
+    m0 = np.array(g.sanity()["mass"][:g.ncomp])
+    g.time_loop(steps)
+    s = g.sanity()
+    assert s["nan_sites"] == 0 and s["negative_sites"] == 0
+    return np.abs(np.array(s["mass"][:g.ncomp]) - m0) / m0
+
+
+def test_binary_256_fp64_mass_and_fp32_against_fp64():
+    """Binary spinodal decomposition, 256^3 periodic."""
+    dims = (256, 256, 256)
+    g64 = lattice(dims, LB3D_FP64, **BINARY)
+    drift = mass_drift(g64, 100)
+    print(f"FP64 256^3 mass drift after 100 steps: {drift}")
+    assert (drift <= 1e-13).all()
+    rho64 = g64.density()
+    g64.close()
+    two, one = lattice(dims, LB3D_FP32_DEV, **BINARY), lattice(dims, LB3D_FP32_DEV, **BINARY)
+    two.set_plan(0)
+    one.set_plan(2, 128, 4, 128, xcd_group=0)  # the benchmark's one-pass plan
+    two.time_loop(100)
+    one.time_loop(100)
+    rho_two, rho_one = two.density(), one.density()
+    assert np.array_equal(rho_two, rho_one)
+    err = np.abs(rho_one - rho64).max() / np.abs(rho64).max()
+    print(f"FP32 vs FP64 density after 100 steps: Linf {err:.3e}")
+    assert err <= 1e-5
+
+
+def test_ternary_128_mass_plans_and_fp32():
+    """Ternary amphiphilic mixture, 128^3 periodic."""
+    dims = (128, 128, 128)
+    g64 = lattice(dims, LB3D_FP64, **TERNARY)
+    drift = mass_drift(g64, 100)
+    print(f"FP64 ternary 128^3 mass drift after 100 steps: {drift}")
+    # the reference's own FP64 arithmetic drifts the surfactant mass by about
+    # 1e-12 per 100 steps (the oracle: 6e-13 at 32^3, 2e-12 at 48^3, seed 1);
+    # red and blue stay at the 1e-14 level
+    assert (drift[:2] <= 1e-13).all() and drift[2] <= 1e-11
+    rho64 = g64.density()
+    g64.close()
+    three, one = lattice(dims, LB3D_FP32_DEV, **TERNARY), lattice(dims, LB3D_FP32_DEV, **TERNARY)
+    three.set_plan(0)
+    one.set_plan(3, 32, 8, 16, xcd_group=0)
+    three.time_loop(100)
+    one.time_loop(100)
+    rho3, rho1 = three.density(), one.density()
+    assert np.array_equal(rho3, rho1)
+    err = np.abs(rho1 - rho64).max() / np.abs(rho64).max()
+    print(f"FP32 vs FP64 ternary density after 100 steps: Linf {err:.3e}")
+    assert err <= 1e-5
+
+
+def test_lees_edwards_256x64x64_mass_and_plans():
+    """The sheared binary mixture at the benchmark's size (shear planes normal
+    to x, wall speed 0.05)."""
+    dims = (256, 64, 64)
+    kw = dict(BINARY, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.05)
+    g64 = lattice(dims, LB3D_FP64, **kw)
+    drift = mass_drift(g64, 200)
+    print(f"FP64 Lees-Edwards mass drift after 200 steps: {drift}")
+    assert (drift <= 1e-12).all()
+    g64.close()
+    two, one = lattice(dims, LB3D_FP32_DEV, **kw), lattice(dims, LB3D_FP32_DEV, **kw)
+    two.set_plan(0)
+    one.set_plan(1, 64, 8, 64, xcd_group=0)
+    two.time_loop(50)
+    one.time_loop(50)
+    assert np.array_equal(two.density(), one.density())
+    assert np.array_equal(two.le_halo(), one.le_halo())
+
+
+def test_batched_ensemble_48_equals_single_lattices():
+    """The 48-lattice ternary ensemble (edges uniform in {16, ..., 96}, g_br in
+    [0.06, 0.12]) in persistent FP32 launches; the largest and two other
+    lattices against single-lattice runs, bitwise."""
+    rng = np.random.default_rng(1)
+    edges = [int(e) for e in 16 + 8 * rng.integers(0, 11, 48)]
+    g_br = [float(v) for v in 0.06 + 0.06 * rng.random(48)]
+    common = {k: v for k, v in TERNARY.items() if k != "g_br"}
+    b = Batch([(e, e, e) for e in edges], per_lattice=[dict(g_br=v) for v in g_br],
+              precision=LB3D_FP32_DEV, **common)
+    for k in range(48):
+        b.init_noise(k, 1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+    b.autotune(2)
+    b.time_loop(10)
+    for k in sorted({int(np.argmax(edges)), 0, 47}):
+        g = Lattice(edges[k], edges[k], edges[k], precision=LB3D_FP32_DEV, g_br=g_br[k], **common)
+        g.init_noise(1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+        g.bc_init()
+        g.time_loop(10)
+        np.testing.assert_array_equal(b.populations(k), g.populations(), err_msg=f"lattice {k} edge {edges[k]}")
+        g.close()
+    b.close()
+
+
+def test_ternary_at_the_indexing_limit():
+    """19 x 3 x sites must stay below 2^32: 422^3 is the largest ternary cube
+    (75.2M sites, 34 GB of FP32 populations); 423^3 is refused."""
+    with pytest.raises(LB3DError):
+        Lattice(423, 423, 423, precision=LB3D_FP32_DEV, **TERNARY)
+    g = lattice((422, 422, 422), LB3D_FP32_DEV, **TERNARY)
+    drift = mass_drift(g, 2)
+    print(f"422^3 ternary FP32: mass drift after 2 steps {drift}")
+    assert (drift <= 1e-6).all()
+    rho = g.density()
+    assert np.isfinite(rho).all() and rho.min() > 0.5 and rho.max() < 0.9
+    g.close()
