@@ -213,6 +213,24 @@ def measure_traffic(n, plan, kernel_key, timeout=150):
                 write_scale=round(write_scale, 4), raw_kib=dict(fetch=kf, write=kw)), None
 
 
+def hbm_ceiling(nbytes=2 << 30, reps=10):
+    """The card's practical HBM rates from lb3d_amd/liblb3d_bwprobe.so: best
+    of a few grid sizes of a 16-byte-per-lane copy (read + write bytes), a
+    read-only and a write-only sweep over nbytes per buffer, in GB/s."""
+    import ctypes
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lb3d_amd", "liblb3d_bwprobe.so")
+    lib = ctypes.CDLL(path)
+    lib.lb3d_bw_probe.restype = ctypes.c_int
+    lib.lb3d_bw_probe.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = (ctypes.c_double * 3)()
+    rc = lib.lb3d_bw_probe(nbytes, reps, out)
+    if rc:
+        raise RuntimeError(f"lb3d_bw_probe failed with HIP error {rc}")
+    return dict(copy=round(out[0], 1), read=round(out[1], 1), write=round(out[2], 1), unit="GB/s",
+                probe=f"float4 grid-stride sweeps over {nbytes >> 20} MiB buffers, best of 4 grid sizes")
+
+
 def ensemble_spec(nlat=48, seed=1):
     """The batched ternary ensemble: cube edges uniform in {16, 24, ..., 96},
     g_br uniform in [0.06, 0.12], noise seed 1000+k (numpy default_rng(seed))."""
@@ -403,6 +421,13 @@ def main():
         else:
             roofline["traffic_note"] = why
             print(f"[bench] traffic not measured: {why}", file=sys.stderr, flush=True)
+    print("[bench] HBM ceiling probe", file=sys.stderr, flush=True)
+    ceiling = hbm_ceiling()
+    # the dominant kernel against what a plain copy reaches on this card
+    ceiling["kernel_algorithmic_frac_of_copy"] = round(achieved / ceiling["copy"], 4)
+    if roofline["traffic"]:
+        ceiling["kernel_traffic_frac_of_copy"] = round(roofline["traffic"] / (kc * 1e-3) / 1e9 / ceiling["copy"], 4)
+    print(f"[bench] ceiling {ceiling}", file=sys.stderr, flush=True)
     fp64 = dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
                 step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
                 kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()}, plan=r64["plan"])
@@ -445,6 +470,7 @@ def main():
                                "rho=0.7+-0.01 (seed 1), FP32 deviation storage",
                    "lattice": [n, n, n], "components": ncomp, "bytes_per_site": B32},
         "roofline": roofline,
+        "hbm_ceiling": ceiling,
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
         "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),

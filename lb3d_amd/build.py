@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblb3d_gpu.so")
+PROBE = os.path.join(HERE, "liblb3d_bwprobe.so")  # bench.py's HBM ceiling probe
 ARCH = os.environ.get("LB3D_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -52,6 +53,9 @@ def build_hip(jobs=4):
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    tmp = PROBE + ".tmp"
+    run([HIPCC, os.path.join(CSRC, "lb3d_bwprobe.hip"), "-shared", "-o", tmp] + COMMON)
+    os.replace(tmp, PROBE)
     return LIB
 
 
