@@ -157,6 +157,23 @@ def traffic_probe(n, plan):
 COPY_BYTES = 1 << 30
 
 
+def ternary_probe(n):
+    """Child run for rocprofv3 counter passes: 3 steps of the FP32 ternary
+    n^3 lattice on the three-pass plan, then 3 on the one-pass plan (32x8
+    tiles, z-chunks of 16)."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, Lattice
+
+    g = Lattice(n, n, n, ncomp=3, precision=LB3D_FP32_DEV, g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0,
+                beta=10.0, d_0=1.0, bdist=2)
+    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+    g.bc_init()
+    for plan in ((0,), (3, 32, 8, 16, 0)):
+        g.set_plan(*plan)
+        g.time_loop(3)
+        g.synchronize()
+    g.close()
+
+
 def counter_means(outdir, counter):
     """Mean value per dispatch of `counter`, by kernel name, from a rocprofv3 database."""
     import glob
@@ -342,6 +359,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
     ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--ternary-probe", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -358,6 +376,9 @@ def main():
         return
     if args.traffic_probe:
         traffic_probe(args.size, json.loads(args.traffic_probe))
+        return
+    if args.ternary_probe:
+        ternary_probe(args.ternary_probe)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
