@@ -138,7 +138,6 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
     b->prec = p0.precision;
     b->rs = b->prec == LB3D_FP64 ? 8 : 4;
     b->B = b->prec == LB3D_FP64 ? make_batch_launcher_fp64() : make_batch_launcher_fp32();
-    const int naux = b->C == 3 ? NAUX3 : NAUX2;
     uint64_t fo = 0, ao = 0, dbo = 0, maxns = 0;
     std::vector<uint2> items;
     for (int k = 0; k < nlat; k++) {
@@ -150,7 +149,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
         b->aux_off.push_back(ao);
         b->db_off.push_back(dbo);
         fo = round64(fo + (uint64_t)b->C * 19 * n);
-        ao = round64(ao + (uint64_t)naux * n);
+        ao = round64(ao + aux_elems(b->C, n));
         dbo = round64(dbo + (b->C == 3 ? 3 * n : 0));
         if (n > maxns) maxns = n;
         if ((uint64_t)b->C * 19 * n >= 4294967296ull) {

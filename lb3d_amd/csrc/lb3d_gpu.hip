@@ -67,7 +67,7 @@ struct lb3d_ctx {
     double *d_partials = nullptr;
     uint8_t *rock_dev = nullptr;  // rock mask (lattices with walls)
     size_t f_bytes = 0, aux_bytes = 0, db_bytes = 0;
-    int naux = 0;
+    size_t aux_elems = 0;  // elements of the aux field
     Plan plan;
     bool tuned = false;
     bool aux_from_prev = false;  // aux is recomputable from f[1-cur] (fused binary steps)
@@ -238,9 +238,9 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
     c->L = p->precision == LB3D_FP64 ? make_launcher_fp64(*p) : make_launcher_fp32(*p);
     size_t rs = c->L->real_bytes();
     size_t ns = (size_t)p->nx * p->ny * p->nz;
-    c->naux = p->ncomp == 3 ? NAUX3 : NAUX2;
+    c->aux_elems = aux_elems(p->ncomp, ns);
     c->f_bytes = (size_t)p->ncomp * 19 * ns * rs;
-    c->aux_bytes = (size_t)c->naux * ns * rs;
+    c->aux_bytes = c->aux_elems * rs;
     c->db_bytes = p->ncomp == 3 ? 3 * ns * rs : 0;
     size_t plane = (size_t)p->ny * p->nz;
     auto cleanup = [&](int code, const std::string &m) {
@@ -446,7 +446,7 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
         c->aux_valid = true;
         c->aux_from_prev = false;
     }
-    if (c->aux_valid && (rc = fetch(c->b.aux, (size_t)c->naux * ns, aux))) return rc;
+    if (c->aux_valid && (rc = fetch(c->b.aux, c->aux_elems, aux))) return rc;
     if (C == 3 && (rc = fetch(c->b.db, 3 * ns, db))) return rc;
     std::vector<uint8_t> rock;
     if (c->b.rock) {
@@ -483,7 +483,7 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
                 }
                 if (C == 3)
                     for (int a = 0; a < 3; a++) {
-                        S[72 + a] = c->aux_valid ? aux[(size_t)(6 + a) * ns + s] : 0.0;
+                        S[72 + a] = c->aux_valid ? aux[aux3_vec_off(ns) + 4 * s + 1 + a] : 0.0;
                         S[75 + a] = db[(size_t)a * ns + s];
                     }
                 S[rock_off] = 0.0;

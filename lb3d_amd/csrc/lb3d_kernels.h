@@ -27,8 +27,20 @@ namespace lb3d {
 // every other form (1, 3, 4, 5, 6) through boltz_general at run time
 constexpr int BD_GENERAL = 9;
 
+// Moments of the streamed state (the aux field).  Binary: planes psi_r,
+// psi_b, rho_r, rho_b.  Ternary: planes psi_r, psi_b, psi_s, rho_r, rho_b,
+// rho_s, then at a 4-element boundary one 4-vector per site of what the
+// dipole relaxation reads at its 18 neighbours, (rho_b - rho_r, da_xyz).
 constexpr int NAUX2 = 4;
-constexpr int NAUX3 = 9;
+template <typename R>
+struct alignas(4 * sizeof(R)) V4 {
+    R a, b, c, d;
+};
+__host__ __device__ constexpr uint64_t aux3_vec_off(uint64_t ns) { return (6 * ns + 3) / 4 * 4; }
+__host__ __device__ constexpr uint64_t aux_elems(int ncomp, uint64_t ns)
+{
+    return ncomp == 3 ? aux3_vec_off(ns) + 4 * ns : (uint64_t)NAUX2 * ns;
+}
 
 // Work-groups are dispatched round-robin over the chip's 8 XCDs (XCD =
 // work-group index mod 8), each XCD with its own L2.  xcd_tile maps the
@@ -322,9 +334,8 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
         st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
     }
     if constexpr (C == 3) {
-        st_plane(aux + (size_t)6 * P.ns, s, da[0]);
-        st_plane(aux + (size_t)7 * P.ns, s, da[1]);
-        st_plane(aux + (size_t)8 * P.ns, s, da[2]);
+        V4<R> *vec = (V4<R> *)(aux + aux3_vec_off(P.ns));
+        vec[s] = V4<R>{rho[1] - rho[0], da[0], da[1], da[2]};
     }
 }
 
@@ -390,18 +401,19 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
+    const V4<R> *vec = (const V4<R> *)(aux + aux3_vec_off(P.ns));
+    // the 18 neighbours' (rho_b - rho_r, da): one vector load each
+    V4<R> nb[18];
+#pragma unroll
+    for (int i = 0; i < 18; i++)
+        nb[i] = vec[sidx(wrap1(x + CX[i], P.nx), wrap1(y + CY[i], P.ny), wrap1(z + CZ[i], P.nz), P.nx, P.ny)];
+    const V4<R> own = vec[s];
     R out[3];
     dipole_core<R, Dev>(
-        P, ld_plane(plane_of(aux, 6u, P.ns), s), ld_plane(plane_of(aux, 7u, P.ns), s),
-        ld_plane(plane_of(aux, 8u, P.ns), s), ld_plane(plane_of(aux, 5u, P.ns), s),
-        [&](int i) __attribute__((always_inline)) {
-            int xa = x + CX[i], ya = y + CY[i], za = z + CZ[i];
-            R rr = aux_at<R, false>(aux, nullptr, P, 3, xa, ya, za);
-            R rb = aux_at<R, false>(aux, nullptr, P, 4, xa, ya, za);
-            return rb - rr;
-        },
+        P, own.b, own.c, own.d, ld_plane(plane_of(aux, 5u, P.ns), s),
+        [&](int i) __attribute__((always_inline)) { return nb[i].a; },
         [&](int i, int a) __attribute__((always_inline)) {
-            return aux_at<R, false>(aux, nullptr, P, 6 + a, x + CX[i], y + CY[i], z + CZ[i]);
+            return a == 0 ? nb[i].b : (a == 1 ? nb[i].c : nb[i].d);
         },
         out);
     st_plane(db, s, out[0]);
