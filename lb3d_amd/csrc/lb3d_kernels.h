@@ -846,7 +846,10 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
 {
     constexpr int T = TX * TY, LX = TX + 2, LY = TY + 2, LP = LX * LY;
     constexpr int NH = 2 * LX + 2 * TY;  // halo positions per plane
-    __shared__ R ring[3][2][LP];         // psi of planes k-1, k, k+1 (tile + halo)
+    // psi of the tile + halo, plane p in slot p - z0 + 1 (mod 4): the collide of
+    // plane k reads planes k-1..k+1 while a thread that is done with it may
+    // already fill plane k+2, so one barrier per plane suffices
+    __shared__ R ring[4][2][LP];
     // populations of plane k+1: in LDS, one column per thread (SM 1), or in registers (SM 0)
     __shared__ R stage[SM == 1 ? 38 : 1][SM == 1 ? T : 1];
     R nxt[2][19];
@@ -929,8 +932,8 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     // prologue: planes z0-1 and z0
     fill(z0 - 1, 0);
     fill(z0, 1);
-    int sm = 0, s0 = 1, sp = 2;  // ring slots of planes k-1, k, k+1
     for (int k = z0; k < zend; k++) {
+        const int sm = (k - z0) & 3, s0 = (k - z0 + 1) & 3, sp = (k - z0 + 2) & 3;  // planes k-1, k, k+1
         // the populations of plane k from the stage (own column only)
         R cur[2][19];
         if constexpr (SM != 2) {
@@ -983,11 +986,6 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             else
                 bgk_site<R, Dev, 2, 2>(P, cur, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
         }
-        __syncthreads();
-        int t = sm;
-        sm = s0;
-        s0 = sp;
-        sp = t;
     }
 }
 

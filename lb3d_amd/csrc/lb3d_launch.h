@@ -324,7 +324,7 @@ class LauncherT : public Launcher {
     template <int TX, int TY>
     static constexpr bool fits()
     {
-        return (38 * TX * TY + 6 * (TX + 2) * (TY + 2)) * sizeof(R) <= 96 * 1024;
+        return (38 * TX * TY + 8 * (TX + 2) * (TY + 2)) * sizeof(R) <= 160 * 1024;
     }
     // stage: 0 registers, 1 LDS, 2 reload (k_fused2's SM)
     template <int TX, int TY>
