@@ -113,6 +113,18 @@ __device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__
     return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), sidx(xs, ys, zs, P.nx, P.ny));
 }
 
+// Pull through one base pointer: element (c*19 + i)*ns + site, addressed as a
+// 32-bit byte offset.  The plane offset is a scalar product instead of a
+// 64-bit base per population plane, which a kernel holding the moments of
+// several planes at once could not keep in scalar registers.  Valid while the
+// populations take less than 4 GiB.
+template <typename R, int C>
+__device__ __forceinline__ R pull_flat(const R *__restrict__ f, const Phys<R> &P, int c, int i, int x, int y, int z)
+{
+    int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny), zs = wrap1(z - CZ[i], P.nz);
+    return ld_plane(f, ((uint32_t)c * 19u + (uint32_t)i) * P.ns + sidx(xs, ys, zs, P.nx, P.ny));
+}
+
 // Pull with walls (periodic binary lattices with rock sites): a rock site
 // keeps its own populations (code/lb3d_advection.F90:126), and a fluid site
 // whose source x - c_i is rock takes its own n_bounce(i) instead, the
@@ -264,7 +276,7 @@ __device__ __forceinline__ void zou_he_planes(const Phys<R> &P, R (&n)[19], int 
     if (z == P.nz - 1) zou_he_site<R, Dev>(n, P.zh_flux[c], P.zh_type, false, P.rho0[c]);
 }
 
-template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false>
+template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false, bool Flat = false>
 __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, int x, int y, int z, R (&rho)[C],
@@ -306,6 +318,8 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
             R v;
             if constexpr (Rock)
                 v = pull_rock<R, C>(f, rock, P, c, i, x, y, z, self_rock);
+            else if constexpr (Flat)
+                v = pull_flat<R, C>(f, P, c, i, x, y, z);
             else
                 v = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
             rho[c] = rho[c] + v * R(GD[i]);
@@ -1023,15 +1037,6 @@ struct Fused3Dims {
     {
         return (size_t)(3 * 4 * MP + (4 * 3 + 2 + 3 * 3) * DP) * sizeof(R);
     }
-    // waves per SIMD that the LDS leaves room for (160 KiB and 4 SIMDs per
-    // CU): the register bound of the kernel
-    template <typename R>
-    static constexpr int waves()
-    {
-        int wg = (int)((160 * 1024) / lds_bytes<R>());
-        int w = wg * (T / 64) / 4;
-        return w < 1 ? 1 : (w > 8 ? 8 : w);
-    }
     // ring site h in [0, NH) of the moments region: rows 0-1, rows MY-2 and
     // MY-1, then columns 0, 1, MX-2, MX-1 of the rows between
     __device__ static __forceinline__ void ring_site(int h, int &mx, int &my)
@@ -1053,7 +1058,7 @@ struct Fused3Dims {
 };
 
 template <typename R, bool Dev, int TX, int TY, int BD>
-__global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) void k_fused3(
+__global__ __launch_bounds__(TX *TY) void k_fused3(
     Phys<R> P, const R *__restrict__ f, const R *__restrict__ db_in, R *__restrict__ db_out,
     R *__restrict__ fout, int lz, uint32_t xg)
 {
@@ -1115,7 +1120,7 @@ __global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) 
             for (int i = 0; i < 19; i++) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    R v = pull<R, 3, false>(f, nullptr, P, c, i, xo, yo, zw);
+                    R v = pull_flat<R, 3>(f, P, c, i, xo, yo, zw);
                     acc_moments<R>(i, v, hr[c], hm[c]);
                     if (c == 2) acc_da<R, Dev>(P, db_in, i, v, xo, yo, zw, da0, da1, da2);
                 }
@@ -1126,7 +1131,7 @@ __global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) 
             int xo = wrap1(x, P.nx), yo = wrap1(y, P.ny);
             asm volatile("" : "+v"(xo), "+v"(yo));
             R rho[3], da[3];
-            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
+            moments_core<R, Dev, 3, false, false, false, true>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
             put(zp, lx + 2, ly + 2, rho, da);
         }
 #pragma unroll
@@ -1140,7 +1145,7 @@ __global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) 
             int xo = wrap1(gxs, P.nx), yo = wrap1(gys, P.ny);
             asm volatile("" : "+v"(xo), "+v"(yo));
             R rho[3], da[3];
-            moments_core<R, Dev, 3, false>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
+            moments_core<R, Dev, 3, false, false, false, true>(P, f, nullptr, db_in, xo, yo, zw, rho, da);
             put(zp, mx, my, rho, da);
         }
     };
@@ -1210,7 +1215,7 @@ __global__ __launch_bounds__(TX *TY, (Fused3Dims<TX, TY>::template waves<R>())) 
             bgk_relax<R, Dev, 3, BD>(
                 P,
                 [&](int c, int i) __attribute__((always_inline)) {
-                    return pull<R, 3, false>(f, nullptr, P, c, i, xo, yo, k);
+                    return pull_flat<R, 3>(f, P, c, i, xo, yo, k);
                 },
                 r0, m0, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
         }

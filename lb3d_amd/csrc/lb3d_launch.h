@@ -393,6 +393,8 @@ class LauncherT : public Launcher {
         if (C == 3) {
             // ternary: the reload form only, periodic lattices, nz >= 3 for the +-2 plane wrap
             if (le || lz < 1 || st != 2 || P.nz < 3 || P.nx < 3 || P.ny < 3) return false;
+            // k_fused3 addresses the populations with 32-bit byte offsets
+            if ((uint64_t)3 * 19 * P.ns * sizeof(R) > 0xffffffffull) return false;
             if (P.bdist != 0 && P.bdist != 2) return false;
             if (tx == 32 && ty == 4) return fused3_t<32, 4>(b, lz, s, launch);
             if (tx == 32 && ty == 8) return fused3_t<32, 8>(b, lz, s, launch);
