@@ -767,11 +767,15 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
             sc_forces_wall<R, Dev>(P, aux, rock, x, y, z, s, F);
         }
     } else {
+        // the force stencil first, then the pulls (a compiler barrier keeps
+        // them after it): the populations are not live while the stencil's
+        // loads are in flight, which leaves the registers to batch those loads
+        sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int i = 0; i < 19; i++)
 #pragma unroll
             for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
-        sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
     }
     if constexpr (ZH) {
         if (z == 0 || z == P.nz - 1) {
