@@ -1216,12 +1216,16 @@ __global__ __launch_bounds__(TX *TY) void k_fused3(
                 F);
             int xo = x, yo = y;
             asm volatile("" : "+v"(xo), "+v"(yo));
+            // the site's populations pulled again, all at once before the
+            // relaxation (its stores would otherwise sit between the loads)
+            R n[3][19];
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) n[c][i] = pull_flat<R, 3>(f, P, c, i, xo, yo, k);
             bgk_relax<R, Dev, 3, BD>(
-                P,
-                [&](int c, int i) __attribute__((always_inline)) {
-                    return pull_flat<R, 3>(f, P, c, i, xo, yo, k);
-                },
-                r0, m0, F, x, y, k, sidx(x, y, k, P.nx, P.ny), fout);
+                P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, r0, m0, F, x, y, k,
+                sidx(x, y, k, P.nx, P.ny), fout);
         }
         __syncthreads();
 #pragma unroll
