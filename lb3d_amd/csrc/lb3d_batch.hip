@@ -337,15 +337,42 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
         if (g < b->nitems) cands.push_back({false, g, 0});
     cands.push_back({false, xcd_round(b->nitems), 0});
-    void *db_save = nullptr;
-    size_t db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
-    if (db_bytes) {
-        BTRY(hipMalloc(&db_save, db_bytes));
-        BTRY(hipMemcpyAsync(db_save, b->db, db_bytes, hipMemcpyDeviceToDevice, b->stream));
+    // the trials overwrite the dipole fields (the dipole pass writes them in
+    // place); the guard restores them and the plan if a trial fails, and
+    // frees the copy and the events on every path
+    struct Guard {
+        lb3d_batch *b;
+        bool seq;
+        unsigned nwg;
+        uint32_t xg;
+        size_t db_bytes = 0;
+        void *db_save = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool restore = true;
+        ~Guard()
+        {
+            if (restore) {
+                b->sequential = seq;
+                b->nwg = nwg;
+                b->B->xg = xg;
+                if (db_save) {
+                    (void)hipMemcpyAsync(b->db, db_save, db_bytes, hipMemcpyDeviceToDevice, b->stream);
+                    (void)hipStreamSynchronize(b->stream);
+                }
+            }
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (db_save) (void)hipFree(db_save);
+        }
+    } g{b, b->sequential, b->nwg, b->xg};
+    g.db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
+    if (g.db_bytes) {
+        BTRY(hipMalloc(&g.db_save, g.db_bytes));
+        BTRY(hipMemcpyAsync(g.db_save, b->db, g.db_bytes, hipMemcpyDeviceToDevice, b->stream));
     }
-    hipEvent_t e0, e1;
-    BTRY(hipEventCreate(&e0));
-    BTRY(hipEventCreate(&e1));
+    BTRY(hipEventCreate(&g.e0));
+    BTRY(hipEventCreate(&g.e1));
+    hipEvent_t e0 = g.e0, e1 = g.e1;
     double best = 1e300;
     Cand bc = {b->sequential, b->nwg, b->xg};
     std::string rep;
@@ -383,13 +410,11 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         for (uint32_t xg : {1u, 4u, 16u, 64u, 256u})
             if (int rc = time_cand({false, top.nwg, xg})) return rc;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (db_save) {
-        BTRY(hipMemcpyAsync(b->db, db_save, db_bytes, hipMemcpyDeviceToDevice, b->stream));
+    if (g.db_save) {
+        BTRY(hipMemcpyAsync(b->db, g.db_save, g.db_bytes, hipMemcpyDeviceToDevice, b->stream));
         BTRY(hipStreamSynchronize(b->stream));
-        (void)hipFree(db_save);
     }
+    g.restore = false;
     b->sequential = bc.seq;
     b->nwg = bc.nwg;
     b->xg = bc.xg;
@@ -412,7 +437,16 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
 {
     clear_error();
     if (!b || nsteps < 1 || !ms_phase || !ms_per_step) return set_error(LB3D_EINVAL, "bad argument");
-    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    // destroyed when the scope ends, also on an early error return
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs{std::vector<hipEvent_t>((size_t)nsteps * 4, nullptr)};
+    std::vector<hipEvent_t> &ev = evs.e;
     for (auto &e : ev) BTRY(hipEventCreate(&e));
     for (int st = 0; st < nsteps; st++) {
         BTRY(hipEventRecord(ev[(size_t)st * 4], b->stream));
@@ -443,7 +477,6 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
             all += m;
         }
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     for (int ph = 0; ph < 3; ph++) ms_phase[ph] = tot[ph] / nsteps;
     *ms_per_step = all / nsteps;
     b->nt += nsteps;

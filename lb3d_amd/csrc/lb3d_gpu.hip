@@ -35,6 +35,28 @@ int fail(int code, const std::string &msg)
 
 enum StageState { IDLE = 0, AFTER_BC6 = 1, AFTER_ADV = 2, AFTER_BC8 = 3 };
 
+// HIP events that are destroyed when the scope ends, also on an early error return
+struct Events {
+    std::vector<hipEvent_t> e;
+    explicit Events(size_t n) : e(n, nullptr) {}
+    Events(const Events &) = delete;
+    Events &operator=(const Events &) = delete;
+    ~Events()
+    {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    hipError_t create()
+    {
+        for (hipEvent_t &x : e) {
+            hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    hipEvent_t &operator[](size_t i) { return e[i]; }
+};
+
 const char *const KNAMES[] = {"le_neighbours", "le_faces_pc", "moments",
                               "le_faces_aux",  "dipole",      "collide", "fused"};
 enum { K_LEN = 0, K_LE6, K_MOM, K_LE8, K_DIP, K_COL, K_FUS, K_COUNT };
@@ -712,9 +734,9 @@ int lb3d_gpu_run_steps_timed(lb3d_ctx *c, int32_t nsteps, double *ms)
     int rc = sync_le_counter(c);
     if (rc) return rc;
     if ((rc = ensure_graph(c))) return rc;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    Events ev(2);
+    HIP_TRY(ev.create());
+    hipEvent_t e0 = ev[0], e1 = ev[1];
     HIP_TRY(hipEventRecord(e0, c->stream));
     rc = lb3d_gpu_run_steps(c, nsteps);
     if (rc == LB3D_OK) {
@@ -724,8 +746,6 @@ int lb3d_gpu_run_steps_timed(lb3d_ctx *c, int32_t nsteps, double *ms)
         HIP_TRY(hipEventElapsedTime(&t, e0, e1));
         *ms = t;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -776,17 +796,37 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
                     cands.push_back(q);
                 }
     }
-    void *db_save = nullptr;
+    // the trials overwrite the dipole field (ternary two-pass: the dipole
+    // pass writes it in place); it is restored from a copy afterwards.  The
+    // guard frees the copy on every path and, when a trial fails, also puts
+    // back the field and the plan.
+    struct Guard {
+        lb3d_ctx *c;
+        Plan saved;
+        void *db_save = nullptr;
+        bool restore = true;
+        ~Guard()
+        {
+            if (restore) {
+                c->plan = saved;
+                c->L->xg = saved.xg;
+                if (db_save) {
+                    (void)hipMemcpyAsync(c->b.db, db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream);
+                    (void)hipStreamSynchronize(c->stream);
+                }
+            }
+            if (db_save) (void)hipFree(db_save);
+        }
+    } g{c, c->plan};
     if (c->p.ncomp == 3) {
-        HIP_TRY(hipMalloc(&db_save, c->db_bytes));
-        HIP_TRY(hipMemcpyAsync(db_save, c->b.db, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMalloc(&g.db_save, c->db_bytes));
+        HIP_TRY(hipMemcpyAsync(g.db_save, c->b.db, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
     }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    Plan saved = c->plan;
+    Events ev(2);
+    HIP_TRY(ev.create());
+    hipEvent_t e0 = ev[0], e1 = ev[1];
     double best = 1e300;
-    Plan bestp = saved;
+    Plan bestp = g.saved;
     std::string rep;
     auto time_plan = [&](const Plan &q, double &per) -> int {
         c->plan = q;
@@ -828,13 +868,11 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         q.xg = xg;
         if (int rc = time_plan(q, per)) return rc;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (db_save) {
-        HIP_TRY(hipMemcpyAsync(c->b.db, db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (g.db_save) {
+        HIP_TRY(hipMemcpyAsync(c->b.db, g.db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
-        (void)hipFree(db_save);
     }
+    g.restore = false;
     c->plan = bestp;
     c->L->xg = bestp.xg;
     c->tuned = true;
@@ -1059,12 +1097,10 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
     if (c->state != IDLE) return fail(LB3D_ESTATE, "profile_steps needs a completed timestep");
     int rc0 = sync_le_counter(c);
     if (rc0) return rc0;
-    std::vector<hipEvent_t> ev((size_t)nsteps * K_COUNT * 2, nullptr);
+    Events ev((size_t)nsteps * K_COUNT * 2 + 2);
     std::vector<int> used((size_t)nsteps * K_COUNT, 0);
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    hipEvent_t t0, t1;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(ev.create());
+    hipEvent_t t0 = ev[ev.e.size() - 2], t1 = ev[ev.e.size() - 1];
     auto rec = [&](int step, int k, bool start) {
         return hipEventRecord(ev[((size_t)step * K_COUNT + k) * 2 + (start ? 0 : 1)], c->stream);
     };
@@ -1127,9 +1163,6 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
             }
     float all = 0.f;
     HIP_TRY(hipEventElapsedTime(&all, t0, t1));
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     int nk = 0;
     for (int k = 0; k < K_COUNT; k++)
         if (cnt[k]) {
