@@ -120,8 +120,6 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
         if (q.ncomp != p0.ncomp || q.precision != p0.precision)
             return set_error(LB3D_EINVAL, "all lattices of a batch share ncomp and precision");
         if (q.boundary != LB3D_BC_PERIODIC) return set_error(LB3D_EINVAL, "batched lattices are periodic");
-        lb3d_ctx *probe = nullptr;
-        (void)probe;
         if (q.ncomp != 2 && q.ncomp != 3) return set_error(LB3D_EINVAL, "ncomp must be 2 or 3");
         if (q.bdist != 0 && q.bdist != 2) return set_error(LB3D_EINVAL, "bdist must be 0 or 2");
         if (q.inv_fluid != 0) return set_error(LB3D_EINVAL, "the batch runs lattices without invasion");

@@ -44,6 +44,20 @@ def test_invalid_parameters_rejected(kw, msg):
     assert e.value.code == 1 and msg in str(e.value)
 
 
+@pytest.mark.parametrize("kw,msg", [
+    (dict(inv_fluid=12), "invasion"),
+    (dict(bdist=4), "bdist"),
+    (dict(boundary=gpu.LB3D_BC_LEES_EDWARDS), "periodic"),
+    (dict(precision=gpu.LB3D_FP32_DEV), "share"),
+])
+def test_batch_invalid_parameters_rejected(kw, msg):
+    """The batch takes periodic lattices of one form with bdist 0 or 2 and no
+    invasion; the checks run before any device call."""
+    with pytest.raises(gpu.LB3DError) as e:
+        gpu.Batch([(8, 8, 8), (6, 6, 6)], per_lattice=[{}, kw], ncomp=2)
+    assert e.value.code == 1 and msg in str(e.value)
+
+
 def test_too_large_lattice_rejected():
     with pytest.raises(gpu.LB3DError):
         gpu.Lattice(2048, 2048, 512, ncomp=3)
