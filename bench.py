@@ -159,15 +159,15 @@ COPY_BYTES = 1 << 30
 
 def ternary_probe(n):
     """Child run for rocprofv3 counter passes: 3 steps of the FP32 ternary
-    n^3 lattice on the three-pass plan, then 3 on the one-pass plan (32x8
-    tiles, z-chunks of 16)."""
+    n^3 lattice on the three-pass plan, then 3 on each of two one-pass plans
+    (32x8 and 64x4 tiles, z-chunks of 16)."""
     from lb3d_amd.gpu import LB3D_FP32_DEV, Lattice
 
     g = Lattice(n, n, n, ncomp=3, precision=LB3D_FP32_DEV, g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0,
                 beta=10.0, d_0=1.0, bdist=2)
     g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
     g.bc_init()
-    for plan in ((0,), (3, 32, 8, 16, 0)):
+    for plan in ((0,), (3, 32, 8, 16, 0), (3, 64, 4, 16, 0)):
         g.set_plan(*plan)
         g.time_loop(3)
         g.synchronize()
