@@ -86,6 +86,25 @@ __device__ __forceinline__ void st_plane(R *__restrict__ plane, uint32_t site, R
 {
     *(R *)((char *)plane + site * (uint32_t)sizeof(R)) = v;
 }
+// Population loads and stores with an optional non-temporal hint (NT): the
+// streamed populations are touched once per pass, and the hint keeps them
+// from displacing the stencil fields that neighbouring sites re-read from L2.
+template <bool NT, typename R>
+__device__ __forceinline__ R ld_pop(const R *__restrict__ plane, uint32_t site)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load((const R *)((const char *)plane + site * (uint32_t)sizeof(R)));
+    else
+        return ld_plane(plane, site);
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void st_pop(R *__restrict__ plane, uint32_t site, R v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, (R *)((char *)plane + site * (uint32_t)sizeof(R)));
+    else
+        st_plane(plane, site, v);
+}
 template <typename R>
 __device__ __forceinline__ const R *plane_of(const R *__restrict__ base, uint32_t k, uint32_t n)
 {
@@ -95,7 +114,7 @@ __device__ __forceinline__ const R *plane_of(const R *__restrict__ base, uint32_
 // Population i of component c pulled into site (x,y,z): the value at
 // (x-c_i, y-c_i, z-c_i).  With Lees-Edwards a source outside 0..nx-1 in x
 // reads the x-halo plane face_pc.
-template <typename R, int C, bool LE>
+template <typename R, int C, bool LE, bool NT = false>
 __device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__ face_pc,
                                   const Phys<R> &P, int c, int i, int x, int y, int z)
 {
@@ -110,7 +129,7 @@ __device__ __forceinline__ R pull(const R *__restrict__ f, const R *__restrict__
     } else {
         xs = wrap1(xs, P.nx);
     }
-    return ld_plane(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), sidx(xs, ys, zs, P.nx, P.ny));
+    return ld_pop<NT>(plane_of(f, (uint32_t)c * 19u + (uint32_t)i, P.ns), sidx(xs, ys, zs, P.nx, P.ny));
 }
 
 // Pull through one base pointer: element (c*19 + i)*ns + site, addressed as a
@@ -623,11 +642,11 @@ __device__ __forceinline__ void acc_moments(int i, R v, R &rho, R (&mom)[3])
     if (CZ[i] != 0) mom[2] = mom[2] + ng * R(CZ[i]);
 }
 
-template <typename R, bool Dev, int C, int BD, typename Get>
+template <typename R, bool Dev, int C, int BD, bool NT = false, typename Get>
 __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
                                           R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout);
 
-template <typename R, bool Dev, int C, int BD, bool Refetch = false, typename Get>
+template <typename R, bool Dev, int C, int BD, bool Refetch = false, bool NT = false, typename Get>
 __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][3], int x, int y, int z,
                                            uint32_t s, R *__restrict__ fout)
 {
@@ -643,12 +662,12 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
         for (int c = 0; c < C; c++) acc_moments<R>(i, get(c, i), rho[c], mom[c]);
     }
     if constexpr (Refetch) asm volatile("" ::: "memory");
-    bgk_relax<R, Dev, C, BD>(P, get, rho, mom, F, x, y, z, s, fout);
+    bgk_relax<R, Dev, C, BD, NT>(P, get, rho, mom, F, x, y, z, s, fout);
 }
 
 // The relaxation of bgk_site_g from the moments (rho, mom) of the streamed
 // populations that get(c, i) returns.
-template <typename R, bool Dev, int C, int BD, typename Get>
+template <typename R, bool Dev, int C, int BD, bool NT, typename Get>
 __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
                                           R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout)
 {
@@ -715,7 +734,7 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
                 R wi = i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0);
                 R deq = wi * rho[c] + rhof[c] * b[i];
                 R v = get(c, i);
-                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - deq));
+                st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - deq));
             }
         } else {
             if constexpr (BD != BD_GENERAL) boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
@@ -723,24 +742,25 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];
                 R v = get(c, i);
-                st_plane(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - neq));
+                st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - neq));
             }
         }
     }
 }
 
-template <typename R, bool Dev, int C, int BD>
+template <typename R, bool Dev, int C, int BD, bool NT = false>
 __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
                                          int x, int y, int z, uint32_t s, R *__restrict__ fout)
 {
-    bgk_site_g<R, Dev, C, BD>(P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, F, x, y,
+    bgk_site_g<R, Dev, C, BD, false, NT>(P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, F, x, y,
                               z, s, fout);
 }
 
 // --------------------------------------------------------------------------
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
-template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false>
+// NT: bit 0 loads, bit 1 stores the populations with the non-temporal hint
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false, int NT = 0>
 __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ aux,
@@ -775,7 +795,7 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
 #pragma unroll
         for (int i = 0; i < 19; i++)
 #pragma unroll
-            for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+            for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE, (NT & 1) != 0>(f, face_pc, P, c, i, x, y, z);
     }
     if constexpr (ZH) {
         if (z == 0 || z == P.nz - 1) {
@@ -794,13 +814,13 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
                 for (int c = 0; c < C; c++) acc_moments<R>(i, n[c][i], rho[c], mom[c]);
 #pragma unroll
             for (int c = 0; c < C; c++) zou_he_planes<R, Dev>(P, n[c], c, z);
-            bgk_relax<R, Dev, C, BD>(
+            bgk_relax<R, Dev, C, BD, (NT & 2) != 0>(
                 P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, rho, mom, F, x, y, z, s,
                 fout);
             return;
         }
     }
-    bgk_site<R, Dev, C, BD>(P, n, F, x, y, z, s, fout);
+    bgk_site<R, Dev, C, BD, (NT & 2) != 0>(P, n, F, x, y, z, s, fout);
 }
 
 template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false>
@@ -813,7 +833,11 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    collide_site<R, Dev, C, LE, BD, Rock, ZH>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
+    // non-temporal populations (measured on MI355X, FP32 and FP64): ternary
+    // loads and stores (128^3: -6 to -8% per step), binary stores (256^3 FP64:
+    // -2%); with walls the plain accesses
+    constexpr int NT = Rock ? 0 : (C == 3 ? 3 : 2);
+    collide_site<R, Dev, C, LE, BD, Rock, ZH, NT>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
 }
 
 // --------------------------------------------------------------------------
@@ -1503,10 +1527,11 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
-        if (s < L.P.ns)
-            if (L.P.bdist == BD)
-                collide_site<R, Dev, C, false, BD>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
-                                                   C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
+        // non-temporal stores of the populations: -2% per step of the
+        // 48-lattice ternary ensemble; non-temporal loads were slower
+        if (s < L.P.ns && L.P.bdist == BD)
+            collide_site<R, Dev, C, false, BD, false, false, 2>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
+                                                               C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
     }
 }
 
