@@ -384,6 +384,21 @@ __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict_
 }
 
 // --------------------------------------------------------------------------
+// c_i . (a0, a1, a2) over the nonzero components of c_i, in x, y, z order.  A
+// zero component's product is an exact zero that leaves the sum unchanged
+// (finite operands), so the terms are skipped instead of multiplied out; the
+// -0.0 start is the additive identity the compiler folds away.
+template <typename R>
+__device__ __forceinline__ R dot_c(int i, R a0, R a1, R a2)
+{
+    R t = R(-0.0);
+    if (CX[i] != 0) t = t + R(CX[i]) * a0;
+    if (CY[i] != 0) t = t + R(CY[i]) * a1;
+    if (CZ[i] != 0) t = t + R(CZ[i]) * a2;
+    return t;
+}
+
+// --------------------------------------------------------------------------
 // k_dipole: dipole relaxation, code/lb3d_collision.F90:296-403 (ternary).
 // The dipole update of one site from its own advected dipole hs = da(x) and
 // surfactant density, and per neighbour i (x + c_i) get_drb(i) = rho_b - rho_r
@@ -405,10 +420,15 @@ __device__ __forceinline__ void dipole_core(const Phys<R> &P, R hs0, R hs1, R hs
         if (CY[i] != 0) hc1 = hc1 + hc_m * R(CY[i]);
         if (CZ[i] != 0) hc2 = hc2 + hc_m * R(CZ[i]);
         R p0 = get_da(i, 0), p1 = get_da(i, 1), p2 = get_da(i, 2);
-        R hs_dot = R(2.0) * (R(CX[i]) * p0 + R(CY[i]) * p1 + R(CZ[i]) * p2);
-        hs0 = hs0 + (p0 - R(CX[i]) * hs_dot) * gs;
-        hs1 = hs1 + (p1 - R(CY[i]) * hs_dot) * gs;
-        hs2 = hs2 + (p2 - R(CZ[i]) * hs_dot) * gs;
+        R hs_dot = R(2.0) * dot_c<R>(i, p0, p1, p2);
+        // (p - c_i hs_dot) g_i; a zero component of c_i leaves p
+        R q0 = p0, q1 = p1, q2 = p2;
+        if (CX[i] != 0) q0 = p0 - R(CX[i]) * hs_dot;
+        if (CY[i] != 0) q1 = p1 - R(CY[i]) * hs_dot;
+        if (CZ[i] != 0) q2 = p2 - R(CZ[i]) * hs_dot;
+        hs0 = hs0 + q0 * gs;
+        hs1 = hs1 + q1 * gs;
+        hs2 = hs2 + q2 * gs;
     }
     R h0 = hc0 + hs0, h1 = hc1 + hs1, h2 = hc2 + hs2;
     R h_m = P.beta * sqrt(h0 * h0 + h1 * h1 + h2 * h2);
@@ -435,12 +455,14 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     const V4<R> *vec = (const V4<R> *)(aux + aux3_vec_off(P.ns));
-    // the 18 neighbours' (rho_b - rho_r, da): one vector load each
+    // the 18 neighbours' (rho_b - rho_r, da): one vector load each, at a
+    // 32-bit byte offset from the field's base (16 ns bytes < 4 GiB for any
+    // ternary lattice that 32-bit population indexing admits)
     V4<R> nb[18];
 #pragma unroll
     for (int i = 0; i < 18; i++)
-        nb[i] = vec[sidx(wrap1(x + CX[i], P.nx), wrap1(y + CY[i], P.ny), wrap1(z + CZ[i], P.nz), P.nx, P.ny)];
-    const V4<R> own = vec[s];
+        nb[i] = ld_plane(vec, sidx(wrap1(x + CX[i], P.nx), wrap1(y + CY[i], P.ny), wrap1(z + CZ[i], P.nz), P.nx, P.ny));
+    const V4<R> own = ld_plane(vec, s);
     R out[3];
     dipole_core<R, Dev>(
         P, own.b, own.c, own.d, ld_plane(plane_of(aux, 5u, P.ns), s),
@@ -486,24 +508,30 @@ __device__ __forceinline__ void sc_forces3(const Phys<R> &P, const R (&dbx)[3], 
         if (CX[i] != 0) { frb[0] = frb[0] + green_r * c0; fbr[0] = fbr[0] + green_b * c0; }
         if (CY[i] != 0) { frb[1] = frb[1] + green_r * c1; fbr[1] = fbr[1] + green_b * c1; }
         if (CZ[i] != 0) { frb[2] = frb[2] + green_r * c2; fbr[2] = fbr[2] + green_b * c2; }
+        // a zero component of c_i makes its terms exact zeros, which are
+        // skipped (dot_c and the conditionals); the rest keep the order
         R d0 = get_db(i, 0) * ps;
         R d1 = get_db(i, 1) * ps;
         R d2 = get_db(i, 2) * ps;
-        R d_dotc = d0 * c0 + d1 * c1 + d2 * c2;
-        fcs[0] = fcs[0] + (d0 - (R(2.0) * d_dotc) * c0) * gs;
-        fcs[1] = fcs[1] + (d1 - (R(2.0) * d_dotc) * c1) * gs;
-        fcs[2] = fcs[2] + (d2 - (R(2.0) * d_dotc) * c2) * gs;
-        R di_dotc = dbx0 * c0 + dbx1 * c1 + dbx2 * c2;
+        R d_dotc = dot_c<R>(i, d0, d1, d2);
+        fcs[0] = fcs[0] + (CX[i] != 0 ? d0 - (R(2.0) * d_dotc) * c0 : d0) * gs;
+        fcs[1] = fcs[1] + (CY[i] != 0 ? d1 - (R(2.0) * d_dotc) * c1 : d1) * gs;
+        fcs[2] = fcs[2] + (CZ[i] != 0 ? d2 - (R(2.0) * d_dotc) * c2 : d2) * gs;
+        R di_dotc = dot_c<R>(i, dbx0, dbx1, dbx2);
         R greenc_s = ((pb - pr) + dpsi_br) * gs;
-        fsc[0] = fsc[0] + greenc_s * (dbx0 - (R(2.0) * di_dotc) * c0);
-        fsc[1] = fsc[1] + greenc_s * (dbx1 - (R(2.0) * di_dotc) * c1);
-        fsc[2] = fsc[2] + greenc_s * (dbx2 - (R(2.0) * di_dotc) * c2);
+        fsc[0] = fsc[0] + greenc_s * (CX[i] != 0 ? dbx0 - (R(2.0) * di_dotc) * c0 : dbx0);
+        fsc[1] = fsc[1] + greenc_s * (CY[i] != 0 ? dbx1 - (R(2.0) * di_dotc) * c1 : dbx1);
+        fsc[2] = fsc[2] + greenc_s * (CZ[i] != 0 ? dbx2 - (R(2.0) * di_dotc) * c2 : dbx2);
         R di_dot_d = d0 * dbx0 + d1 * dbx1 + d2 * dbx2;
         R green_ss = R(8.0) * gs;
         R a = di_dot_d - (R(2.0) * di_dotc) * d_dotc;
-        fss[0] = fss[0] + green_ss * ((a * c0 + d0 * di_dotc) + d_dotc * dbx0);
-        fss[1] = fss[1] + green_ss * ((a * c1 + d1 * di_dotc) + d_dotc * dbx1);
-        fss[2] = fss[2] + green_ss * ((a * c2 + d2 * di_dotc) + d_dotc * dbx2);
+        R q0 = d0 * di_dotc, q1 = d1 * di_dotc, q2 = d2 * di_dotc;
+        if (CX[i] != 0) q0 = a * c0 + d0 * di_dotc;
+        if (CY[i] != 0) q1 = a * c1 + d1 * di_dotc;
+        if (CZ[i] != 0) q2 = a * c2 + d2 * di_dotc;
+        fss[0] = fss[0] + green_ss * (q0 + d_dotc * dbx0);
+        fss[1] = fss[1] + green_ss * (q1 + d_dotc * dbx1);
+        fss[2] = fss[2] + green_ss * (q2 + d_dotc * dbx2);
     }
     R psr = ps_own[0], psb = ps_own[1], pss = ps_own[2];
     if constexpr (Dev) {
