@@ -160,6 +160,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
     int mask = 0;
     for (int k = 0; k < nlat; k++) mask |= 1 << params[k].bdist;
     b->B->set_bdist_mask(mask);
+    b->B->set_pop_bytes(fo * b->rs);
     std::vector<unsigned char> lat((size_t)nlat * b->B->lat_bytes());
     for (int k = 0; k < nlat; k++)
         b->B->fill_lat(lat.data() + (size_t)k * b->B->lat_bytes(), params[k], b->f_off[k], b->aux_off[k], b->db_off[k]);

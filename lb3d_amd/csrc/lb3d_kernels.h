@@ -89,6 +89,10 @@ __device__ __forceinline__ void st_plane(R *__restrict__ plane, uint32_t site, R
 // Population loads and stores with an optional non-temporal hint (NT): the
 // streamed populations are touched once per pass, and the hint keeps them
 // from displacing the stencil fields that neighbouring sites re-read from L2.
+// It pays only when the populations exceed the memory-side cache (MI355X:
+// 256 MiB of Infinity Cache): a smaller lattice written without the hint is
+// read back from that cache by the next step.
+constexpr uint64_t MALL_BYTES = 256ull << 20;
 template <bool NT, typename R>
 __device__ __forceinline__ R ld_pop(const R *__restrict__ plane, uint32_t site)
 {
@@ -851,7 +855,9 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
     bgk_site<R, Dev, C, BD, (NT & 2) != 0>(P, n, F, x, y, z, s, fout);
 }
 
-template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false>
+// NT: the populations' loads and stores with the non-temporal hint, for
+// lattices whose populations do not fit the memory-side cache (see ld_pop)
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false, bool NT = false>
 __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ aux,
@@ -861,11 +867,10 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    // non-temporal populations (measured on MI355X, FP32 and FP64): ternary
-    // loads and stores (128^3: -6 to -8% per step), binary stores (256^3 FP64:
-    // -2%); with walls the plain accesses
-    constexpr int NT = Rock ? 0 : (C == 3 ? 3 : 2);
-    collide_site<R, Dev, C, LE, BD, Rock, ZH, NT>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
+    // measured on MI355X (FP32 and FP64): ternary loads and stores (128^3:
+    // -6 to -8% per step), binary stores only (256^3 FP64: -2%)
+    constexpr int nt = !NT ? 0 : (C == 3 ? 3 : 2);
+    collide_site<R, Dev, C, LE, BD, Rock, ZH, nt>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
 }
 
 // --------------------------------------------------------------------------
@@ -1540,7 +1545,7 @@ __global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restr
     }
 }
 
-template <typename R, bool Dev, int C, int BD>
+template <typename R, bool Dev, int C, int BD, bool NT>
 __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__restrict__ lat,
                                                        const uint2 *__restrict__ items,
                                                        uint32_t nitems, const R *__restrict__ f,
@@ -1555,10 +1560,10 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
-        // non-temporal stores of the populations: -2% per step of the
+        // non-temporal stores of the populations (NT): -2% per step of the
         // 48-lattice ternary ensemble; non-temporal loads were slower
         if (s < L.P.ns && L.P.bdist == BD)
-            collide_site<R, Dev, C, false, BD, false, false, 2>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
+            collide_site<R, Dev, C, false, BD, false, false, NT ? 2 : 0>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
                                                                C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
     }
 }

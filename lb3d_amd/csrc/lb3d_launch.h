@@ -276,39 +276,42 @@ class LauncherT : public Launcher {
     {
         if (C == 3) k_dipole<R, Dev><<<tiles(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db, xg);
     }
-    template <int BD>
+    template <int BD, bool NT>
     void collide_bd(const DevBufs &b, hipStream_t s)
     {
         const R *f = (const R *)b.f[b.cur];
         R *fo = (R *)b.f[1 - b.cur];
         if (P.zh && C == 3)
-            k_collide<R, Dev, 3, false, BD, false, true><<<tiles(), BLOCK, 0, s>>>(
+            k_collide<R, Dev, 3, false, BD, false, true, NT><<<tiles(), BLOCK, 0, s>>>(
                 P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db, fo, xg, nullptr);
         else if (P.zh && !le && !b.rock)
-            k_collide<R, Dev, 2, false, BD, false, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
-                                                                                 nullptr, nullptr, fo, xg, nullptr);
+            k_collide<R, Dev, 2, false, BD, false, true, NT><<<tiles(), BLOCK, 0, s>>>(
+                P, f, nullptr, (const R *)b.aux, nullptr, nullptr, fo, xg, nullptr);
         else if (C == 3)
-            k_collide<R, Dev, 3, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      (const R *)b.db, fo, xg, nullptr);
+            k_collide<R, Dev, 3, false, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
+                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db, fo, xg, nullptr);
         else if (le)
-            k_collide<R, Dev, 2, true, BD><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc,
-                                                                     (const R *)b.aux, (const R *)b.face_aux,
-                                                                     nullptr, fo, xg, nullptr);
+            k_collide<R, Dev, 2, true, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
+                P, f, (const R *)b.face_pc, (const R *)b.aux, (const R *)b.face_aux, nullptr, fo, xg, nullptr);
         else if (b.rock)
             k_collide<R, Dev, 2, false, BD, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
                                                                             nullptr, nullptr, fo, xg, b.rock);
         else
-            k_collide<R, Dev, 2, false, BD><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux, nullptr,
-                                                                      nullptr, fo, xg, nullptr);
+            k_collide<R, Dev, 2, false, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
+                P, f, nullptr, (const R *)b.aux, nullptr, nullptr, fo, xg, nullptr);
     }
+    // the non-temporal form once one copy of the populations exceeds the
+    // memory-side cache
+    bool nt() const { return (uint64_t)C * 19u * P.ns * sizeof(R) > MALL_BYTES; }
     void collide(const DevBufs &b, hipStream_t s) override
     {
+        const bool n = nt();
         if (P.bdist == 0)
-            collide_bd<0>(b, s);
+            n ? collide_bd<0, true>(b, s) : collide_bd<0, false>(b, s);
         else if (P.bdist == 2)
-            collide_bd<2>(b, s);
+            n ? collide_bd<2, true>(b, s) : collide_bd<2, false>(b, s);
         else
-            collide_bd<BD_GENERAL>(b, s);
+            n ? collide_bd<BD_GENERAL, true>(b, s) : collide_bd<BD_GENERAL, false>(b, s);
     }
     void le_faces(const DevBufs &b, bool post, hipStream_t s) override
     {
@@ -461,7 +464,9 @@ template <typename R, bool Dev>
 class BatchLauncherT : public BatchLauncher {
   public:
     int bd_mask = 0;  // equilibrium forms present in the batch (bit 0: bdist 0, bit 2: bdist 2)
+    bool nt = false;  // non-temporal population stores: the populations exceed the memory-side cache
     void set_bdist_mask(int m) override { bd_mask = m; }
+    void set_pop_bytes(uint64_t bytes) override { nt = bytes > MALL_BYTES; }
     size_t lat_bytes() const override { return sizeof(BatchLat<R>); }
     void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                   uint64_t db_off) const override
@@ -489,20 +494,29 @@ class BatchLauncherT : public BatchLauncher {
             // every lattice's collide runs in the launch of its own equilibrium form
             for (int bd : {0, 2}) {
                 if (!(bd_mask & (1 << bd))) continue;
-                if (ncomp == 3 && bd == 0)
-                    k_batch_collide<R, Dev, 3, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         (const R *)db, (R *)fout, xg);
-                else if (ncomp == 3)
-                    k_batch_collide<R, Dev, 3, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         (const R *)db, (R *)fout, xg);
-                else if (bd == 0)
-                    k_batch_collide<R, Dev, 2, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         nullptr, (R *)fout, xg);
+                if (nt)
+                    collide_bd<true>(bd, L, it, nitems, nwg, f, fout, aux, db, ncomp, s);
                 else
-                    k_batch_collide<R, Dev, 2, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                         nullptr, (R *)fout, xg);
+                    collide_bd<false>(bd, L, it, nitems, nwg, f, fout, aux, db, ncomp, s);
             }
         }
+    }
+    template <bool NT>
+    void collide_bd(int bd, const BatchLat<R> *L, const uint2 *it, uint32_t nitems, unsigned nwg, const void *f,
+                    void *fout, void *aux, void *db, int ncomp, hipStream_t s)
+    {
+        if (ncomp == 3 && bd == 0)
+            k_batch_collide<R, Dev, 3, 0, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                     (const R *)db, (R *)fout, xg);
+        else if (ncomp == 3)
+            k_batch_collide<R, Dev, 3, 2, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                     (const R *)db, (R *)fout, xg);
+        else if (bd == 0)
+            k_batch_collide<R, Dev, 2, 0, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                     nullptr, (R *)fout, xg);
+        else
+            k_batch_collide<R, Dev, 2, 2, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                     nullptr, (R *)fout, xg);
     }
     void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,
               void *aux, void *db, int ncomp, hipStream_t s) override

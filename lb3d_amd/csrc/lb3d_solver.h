@@ -70,6 +70,8 @@ class BatchLauncher {
     uint32_t xg = 0;  // chunk of consecutive work items per XCD (xcd_tile)
     virtual size_t lat_bytes() const = 0;   // sizeof(BatchLat<R>)
     virtual void set_bdist_mask(int mask) = 0;
+    // bytes of one copy of the batch's populations (chooses the collide's store form)
+    virtual void set_pop_bytes(uint64_t bytes) = 0;
     // host image of the lattice table entry k
     virtual void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                           uint64_t db_off) const = 0;
