@@ -267,8 +267,17 @@ int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double 
  * or one lattice after another with the single-lattice kernels
  * (sequential = 1). */
 int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int32_t xcd_group);
-/* Autotuner: times the sequential plan and persistent grid sizes on the
- * batch's own state (unchanged) and keeps the fastest. */
+/* Groups of the persistent plan: consecutive lattices whose populations add
+ * up to about group_bytes (a larger lattice alone) run their moments, dipole
+ * and collide phases before the next group starts, so that the collide finds
+ * the group's populations and moments in the memory-side cache (256 MiB on
+ * MI355X).  0 = one group: each phase over the whole batch.  The results do
+ * not depend on the grouping.  No counterpart in the reference (an execution
+ * plan of the batched ensemble, which the reference does not have). */
+int lb3d_gpu_batch_set_groups(lb3d_batch *b, uint64_t group_bytes);
+/* Autotuner: times the sequential plan, persistent grid sizes, XCD chunks
+ * and lattice groups on the batch's own state (unchanged) and keeps the
+ * fastest. */
 int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, int32_t report_len);
 /* Mean HIP-event duration of the moments, dipole and collide phases. */
 int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[3],

@@ -41,6 +41,13 @@ struct lb3d_batch {
     unsigned nwg = 2048;     // persistent grid size, a multiple of 8
     uint32_t xg = 0;         // work items per XCD chunk (xcd_tile)
     bool sequential = false; // one lattice after another with the single-lattice kernels
+    // grouped persistent plan: consecutive lattices whose populations add up
+    // to about group_bytes run their three phases before the next group, so
+    // that the collide re-reads populations and moments that the moments
+    // phase has just brought into the memory-side cache (0: one group)
+    uint64_t group_bytes = 0;
+    std::vector<uint32_t> item0;   // first work item of each lattice, then nitems
+    std::vector<uint32_t> groups;  // first work item of each group, then nitems
 };
 
 namespace {
@@ -57,6 +64,34 @@ DevBufs view(const lb3d_batch *b, int k)
     return v;
 }
 
+// Split the lattices into groups of consecutive lattices holding about
+// group_bytes of populations each (a larger lattice forms its own group).
+void make_groups(lb3d_batch *b)
+{
+    b->groups.assign(1, 0u);
+    if (b->group_bytes) {
+        uint64_t acc = 0;
+        for (int k = 0; k < b->nlat; k++) {
+            uint64_t by = (uint64_t)b->C * 19u * b->ns[k] * b->rs;
+            if (acc > 0 && acc + by > b->group_bytes) {
+                b->groups.push_back(b->item0[k]);
+                acc = 0;
+            }
+            acc += by;
+        }
+    }
+    b->groups.push_back(b->nitems);
+}
+
+// Phase ph of the persistent plan over the work items of group g.
+void group_phase(lb3d_batch *b, size_t g, int ph)
+{
+    const uint32_t i0 = b->groups[g], n = b->groups[g + 1] - i0;
+    const unsigned nwg = b->nwg < xcd_round(n) ? b->nwg : xcd_round(n);
+    b->B->phase(ph, b->lat, (const uint2 *)b->items + i0, n, nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db,
+                b->C, b->stream);
+}
+
 void enqueue_batch_step(lb3d_batch *b)
 {
     if (b->sequential) {
@@ -67,8 +102,8 @@ void enqueue_batch_step(lb3d_batch *b)
             b->L[k]->collide(v, b->stream);
         }
     } else {
-        b->B->step(b->lat, b->items, b->nitems, b->nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db, b->C,
-                   b->stream);
+        for (size_t g = 0; g + 1 < b->groups.size(); g++)
+            for (int ph = 0; ph < 3; ph++) group_phase(b, g, ph);
     }
     b->cur = 1 - b->cur;
 }
@@ -154,9 +189,12 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
             lb3d_gpu_batch_destroy(b);
             return set_error(LB3D_EINVAL, "a lattice is too large for 32-bit population indexing");
         }
+        b->item0.push_back((uint32_t)items.size());
         for (uint64_t s0 = 0; s0 < n; s0 += BLOCK) items.push_back(make_uint2((unsigned)k, (unsigned)s0));
     }
     b->nitems = (uint32_t)items.size();
+    b->item0.push_back(b->nitems);
+    make_groups(b);
     int mask = 0;
     for (int k = 0; k < nlat; k++) mask |= 1 << params[k].bdist;
     b->B->set_bdist_mask(mask);
@@ -319,6 +357,15 @@ int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int3
     return drop_graph(b);
 }
 
+int lb3d_gpu_batch_set_groups(lb3d_batch *b, uint64_t group_bytes)
+{
+    clear_error();
+    if (!b) return set_error(LB3D_EINVAL, "NULL batch");
+    b->group_bytes = group_bytes;
+    make_groups(b);
+    return drop_graph(b);
+}
+
 // Times the persistent launches over grid sizes, and the lattice-by-lattice
 // sequence, on the batch's own state; trial steps write f[1-cur] without
 // swapping (the ternary dipole field is saved and restored).
@@ -331,11 +378,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         bool seq;
         unsigned nwg;
         uint32_t xg;
+        uint64_t gb;  // group bytes of the grouped persistent plan (0: one group)
     };
-    std::vector<Cand> cands = {{true, 0, 0}};
+    std::vector<Cand> cands = {{true, 0, 0, 0}};
     for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
-        if (g < b->nitems) cands.push_back({false, g, 0});
-    cands.push_back({false, xcd_round(b->nitems), 0});
+        if (g < b->nitems) cands.push_back({false, g, 0, 0});
+    cands.push_back({false, xcd_round(b->nitems), 0, 0});
     // the trials overwrite the dipole fields (the dipole pass writes them in
     // place); the guard restores them and the plan if a trial fails, and
     // frees the copy and the events on every path
@@ -344,6 +392,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         bool seq;
         unsigned nwg;
         uint32_t xg;
+        uint64_t gb;
         size_t db_bytes = 0;
         void *db_save = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -354,6 +403,8 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
                 b->sequential = seq;
                 b->nwg = nwg;
                 b->B->xg = xg;
+                b->group_bytes = gb;
+                make_groups(b);
                 if (db_save) {
                     (void)hipMemcpyAsync(b->db, db_save, db_bytes, hipMemcpyDeviceToDevice, b->stream);
                     (void)hipStreamSynchronize(b->stream);
@@ -363,7 +414,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
             if (e1) (void)hipEventDestroy(e1);
             if (db_save) (void)hipFree(db_save);
         }
-    } g{b, b->sequential, b->nwg, b->xg};
+    } g{b, b->sequential, b->nwg, b->xg, b->group_bytes};
     g.db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
     if (g.db_bytes) {
         BTRY(hipMalloc(&g.db_save, g.db_bytes));
@@ -373,12 +424,14 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     BTRY(hipEventCreate(&g.e1));
     hipEvent_t e0 = g.e0, e1 = g.e1;
     double best = 1e300;
-    Cand bc = {b->sequential, b->nwg, b->xg};
+    Cand bc = {b->sequential, b->nwg, b->xg, b->group_bytes};
     std::string rep;
     auto time_cand = [&](const Cand &c) -> int {
         b->sequential = c.seq;
         b->nwg = c.nwg;
         b->B->xg = c.xg;
+        b->group_bytes = c.gb;
+        make_groups(b);
         auto trial = [&]() {
             enqueue_batch_step(b);
             b->cur = 1 - b->cur;  // undo the swap: the state stays in f[cur]
@@ -392,9 +445,10 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         float ms = 0.f;
         BTRY(hipEventElapsedTime(&ms, e0, e1));
         double per = ms / trial_steps;
-        char buf[96];
-        std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u %.4f ms\n", c.seq ? "sequential" : "persistent", c.nwg,
-                      c.xg, per);
+        char buf[128];
+        std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
+                      c.seq ? "sequential" : "persistent", c.nwg, c.xg, c.seq ? (size_t)0 : b->groups.size() - 1,
+                      (unsigned long long)(c.gb >> 20), per);
         rep += buf;
         if (per < best) {
             best = per;
@@ -407,7 +461,17 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     if (!bc.seq) {  // the XCD chunking of the best persistent grid
         const Cand top = bc;
         for (uint32_t xg : {1u, 4u, 16u, 64u, 256u})
-            if (int rc = time_cand({false, top.nwg, xg})) return rc;
+            if (int rc = time_cand({false, top.nwg, xg, 0})) return rc;
+    }
+    if (!bc.seq) {  // then groups of lattices sized for the memory-side cache
+        const Cand top = bc;
+        uint64_t total = 0;
+        for (int k = 0; k < b->nlat; k++) total += (uint64_t)b->C * 19u * b->ns[k] * b->rs;
+        // measured on the 48-lattice ensemble: 192-256 MiB -1.4% per step,
+        // smaller groups lose more to their launches than they gain
+        for (uint64_t mib : {192u, 256u, 320u})
+            if ((mib << 20) < total)
+                if (int rc = time_cand({false, top.nwg, top.xg, mib << 20})) return rc;
     }
     if (g.db_save) {
         BTRY(hipMemcpyAsync(b->db, g.db_save, g.db_bytes, hipMemcpyDeviceToDevice, b->stream));
@@ -418,10 +482,13 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     b->nwg = bc.nwg;
     b->xg = bc.xg;
     b->B->xg = bc.xg;
+    b->group_bytes = bc.gb;
+    make_groups(b);
     drop_graph(b);
-    char buf[96];
-    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u %.4f ms\n", bc.seq ? "sequential" : "persistent",
-                  bc.nwg, bc.xg, best);
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
+                  bc.seq ? "sequential" : "persistent", bc.nwg, bc.xg, bc.seq ? (size_t)0 : b->groups.size() - 1,
+                  (unsigned long long)(bc.gb >> 20), best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
@@ -430,12 +497,16 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     return LB3D_OK;
 }
 
-// Mean duration per launch of the three phases (persistent plan) over
-// nsteps timesteps, HIP events on the batch's stream.
+// Time per step spent in each of the three phases, summed over the groups
+// of the persistent plan (or the lattices of the sequential plan) in the
+// order the step runs them, over nsteps timesteps: HIP events on the batch's
+// stream after every launch.
 int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[3], double *ms_per_step)
 {
     clear_error();
     if (!b || nsteps < 1 || !ms_phase || !ms_per_step) return set_error(LB3D_EINVAL, "bad argument");
+    const size_t units = b->sequential ? (size_t)b->nlat : b->groups.size() - 1;
+    const size_t per = 1 + 3 * units;  // events per step
     // destroyed when the scope ends, also on an early error return
     struct Events {
         std::vector<hipEvent_t> e;
@@ -444,24 +515,24 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
         }
-    } evs{std::vector<hipEvent_t>((size_t)nsteps * 4, nullptr)};
+    } evs{std::vector<hipEvent_t>((size_t)nsteps * per, nullptr)};
     std::vector<hipEvent_t> &ev = evs.e;
     for (auto &e : ev) BTRY(hipEventCreate(&e));
     for (int st = 0; st < nsteps; st++) {
-        BTRY(hipEventRecord(ev[(size_t)st * 4], b->stream));
-        for (int ph = 0; ph < 3; ph++) {
-            if (b->sequential) {
-                for (int k = 0; k < b->nlat; k++) {
-                    DevBufs v = view(b, k);
-                    if (ph == 0) b->L[k]->moments(v, b->stream);
-                    if (ph == 1) b->L[k]->dipole(v, b->stream);
-                    if (ph == 2) b->L[k]->collide(v, b->stream);
+        size_t q = (size_t)st * per;
+        BTRY(hipEventRecord(ev[q++], b->stream));
+        for (size_t u = 0; u < units; u++) {
+            for (int ph = 0; ph < 3; ph++) {
+                if (b->sequential) {
+                    DevBufs v = view(b, (int)u);
+                    if (ph == 0) b->L[u]->moments(v, b->stream);
+                    if (ph == 1) b->L[u]->dipole(v, b->stream);
+                    if (ph == 2) b->L[u]->collide(v, b->stream);
+                } else {
+                    group_phase(b, u, ph);
                 }
-            } else {
-                b->B->phase(ph, b->lat, b->items, b->nitems, b->nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db,
-                            b->C, b->stream);
+                BTRY(hipEventRecord(ev[q++], b->stream));
             }
-            BTRY(hipEventRecord(ev[(size_t)st * 4 + 1 + ph], b->stream));
         }
         b->cur = 1 - b->cur;
     }
@@ -469,10 +540,10 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
     BTRY(hipStreamSynchronize(b->stream));
     double tot[3] = {0, 0, 0}, all = 0;
     for (int st = 0; st < nsteps; st++) {
-        for (int ph = 0; ph < 3; ph++) {
+        for (size_t j = 0; j + 1 < per; j++) {
             float m = 0.f;
-            BTRY(hipEventElapsedTime(&m, ev[(size_t)st * 4 + ph], ev[(size_t)st * 4 + ph + 1]));
-            tot[ph] += m;
+            BTRY(hipEventElapsedTime(&m, ev[(size_t)st * per + j], ev[(size_t)st * per + j + 1]));
+            tot[j % 3] += m;
             all += m;
         }
     }

@@ -518,11 +518,6 @@ class BatchLauncherT : public BatchLauncher {
             k_batch_collide<R, Dev, 2, 2, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
                                                                      nullptr, (R *)fout, xg);
     }
-    void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,
-              void *aux, void *db, int ncomp, hipStream_t s) override
-    {
-        for (int ph = 0; ph < 3; ph++) phase(ph, lat, items, nitems, nwg, f, fout, aux, db, ncomp, s);
-    }
 };
 
 }  // namespace lb3d

@@ -75,9 +75,7 @@ class BatchLauncher {
     // host image of the lattice table entry k
     virtual void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                           uint64_t db_off) const = 0;
-    virtual void step(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
-                      void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
-    // the three phases separately (for profiling): 0 moments, 1 dipole, 2 collide
+    // phase ph (0 moments, 1 dipole, 2 collide) over nitems work items
     virtual void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg,
                        const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
 };

@@ -40,7 +40,8 @@ EXPORTS = [
     "lb3d_gpu_batch_create", "lb3d_gpu_batch_destroy", "lb3d_gpu_batch_init_noise",
     "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
-    "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_autotune", "lb3d_gpu_batch_profile_steps",
+    "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_set_groups", "lb3d_gpu_batch_autotune",
+    "lb3d_gpu_batch_profile_steps",
 ]
 
 
@@ -138,6 +139,7 @@ def lib():
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
         L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip, ip]
+        L.lb3d_gpu_batch_set_groups.argtypes = [vp, ctypes.c_uint64]
         L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
         _lib = L
@@ -458,6 +460,11 @@ class Batch:
     def set_plan(self, sequential, nwg=2048, xcd_group=0):
         check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg), int(xcd_group)),
               "batch_set_plan")
+
+    def set_groups(self, group_bytes):
+        """Run the persistent plan in groups of lattices of about group_bytes
+        of populations each (0: one group)."""
+        check(lib().lb3d_gpu_batch_set_groups(self._h, int(group_bytes)), "batch_set_groups")
 
     def autotune(self, trial_steps=3):
         buf = ctypes.create_string_buffer(8192)

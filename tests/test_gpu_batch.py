@@ -51,12 +51,18 @@ def test_batch_persistent_equals_sequential_and_single(precision):
             b.init_noise(k, 7 + k, amp=(0.01, 0.01, 0.01))
         return b
 
-    b1, b2, b3, b4 = make(), make(), make(), make()
+    b1, b2, b3, b4, b5, b6 = make(), make(), make(), make(), make(), make()
     b1.set_plan(0, 7)       # persistent, a small grid that strides many times
     b2.set_plan(1)          # lattice by lattice
     b3.autotune(2)          # whatever the tuner picks; the state must be unchanged
     b4.set_plan(0, 24, 3)   # persistent, chunks of 3 work items per XCD
-    for b in (b1, b2, b3, b4):
+    b5.set_plan(0, 16)      # persistent in groups: every lattice its own group
+    b5.set_groups(1)
+    rs = 8 if precision == LB3D_FP64 else 4
+    b6.set_plan(0, 4096)    # groups of about two lattices, one work item per workgroup
+    b6.set_groups(57 * rs * (8 * 8 * 8 + 13 * 7 * 5))
+    bs = (b1, b2, b3, b4, b5, b6)
+    for b in bs:
         b.time_loop(9)
     for k, d in enumerate(DIMS):
         g = Lattice(*d, ncomp=ncomp, precision=precision, **per[k])
@@ -64,7 +70,7 @@ def test_batch_persistent_equals_sequential_and_single(precision):
         g.bc_init()
         g.time_loop(9)
         want = g.populations()
-        for b in (b1, b2, b3, b4):
+        for b in bs:
             np.testing.assert_array_equal(b.populations(k), want)
         sb, sg = b1.sanity(k), g.sanity()
         np.testing.assert_array_equal(sb["max_pos"], sg["max_pos"])
