@@ -414,6 +414,8 @@ class LauncherT : public Launcher {
         if (tx == 32 && ty == 16) return fused_t<32, 16>(b, lz, st, s, launch);
         if (tx == 128 && ty == 4) return fused_t<128, 4>(b, lz, st, s, launch);
         if (tx == 32 && ty == 4) return fused_t<32, 4>(b, lz, st, s, launch);
+        if (tx == 128 && ty == 2) return fused_t<128, 2>(b, lz, st, s, launch);
+        if (tx == 256 && ty == 2) return fused_t<256, 2>(b, lz, st, s, launch);
         return false;
     }
     bool fused(const DevBufs &b, int tx, int ty, int lz, int stage, hipStream_t s) override

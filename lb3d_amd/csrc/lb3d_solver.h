@@ -51,7 +51,8 @@ class Launcher {
 };
 
 // Tile shapes of k_fused2 that are instantiated (TX, TY).
-constexpr int FUSED_SHAPES[][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {128, 4}, {32, 4}, {16, 16}};
+constexpr int FUSED_SHAPES[][2] = {{64, 4}, {32, 8}, {64, 8}, {32, 16}, {128, 4}, {32, 4}, {16, 16},
+                                   {128, 2}, {256, 2}};
 constexpr int N_FUSED_SHAPES = sizeof(FUSED_SHAPES) / sizeof(FUSED_SHAPES[0]);
 
 constexpr int NDIAG = 9;  // mass[3], momentum[3], nan sites, negative sites, fast sites

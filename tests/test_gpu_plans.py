@@ -9,7 +9,7 @@ from tests.test_gpu_parity import assert_pops_close, pair
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 4), (32, 8), (64, 8), (32, 16), (128, 4), (32, 4)]
+SHAPES = [(64, 4), (32, 8), (64, 8), (32, 16), (128, 4), (32, 4), (128, 2), (256, 2)]
 
 
 def fused_shapes(precision):
