@@ -940,11 +940,12 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
     const bool own = x < P.nx && y < P.ny;
     const int zend = min(z0 + lz, P.nz);
     if (z0 >= zend) return;  // uniform over the block
-    // LDS positions this thread fills besides its own: up to two halo slots
-    int hq[2], hgx[2], hgy[2];
-    bool hv[2];
+    // LDS positions this thread fills besides its own: up to NHS halo slots
+    constexpr int NHS = (NH + T - 1) / T;  // halo positions per thread
+    int hq[NHS], hgx[NHS], hgy[NHS];
+    bool hv[NHS];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < NHS; r++) {
         int h = tid + r * T;
         int ax = 0, ay = 0;
         if (h < LX) { ax = h - 1; ay = -1; }
@@ -994,7 +995,7 @@ __global__ __launch_bounds__(TX *TY) void k_fused2(Phys<R> P, const R *__restric
             ring[slot][1][me] = p1;
         }
 #pragma unroll
-        for (int r = 0; r < (NH > T ? 2 : 1); r++)
+        for (int r = 0; r < NHS; r++)
             if (hv[r]) {
                 R p0, p1;
                 int gx = hgx[r], gy = hgy[r];
