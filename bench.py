@@ -174,6 +174,24 @@ def ternary_probe(n):
     g.close()
 
 
+def ensemble_probe(groups_mib):
+    """Child run for rocprofv3 passes: the 48-lattice FP32 ternary ensemble on
+    the persistent plan (one work item per workgroup, XCD chunks of 16 items)
+    in groups of groups_mib MiB of populations (0: one group), 3 steps."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, Batch
+
+    edges, g_br = ensemble_spec(48)
+    b = Batch([(e, e, e) for e in edges], per_lattice=[dict(g_br=g) for g in g_br], precision=LB3D_FP32_DEV,
+              ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
+    for k in range(48):
+        b.init_noise(k, 1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+    b.set_plan(0, nwg=1 << 30, xcd_group=16)
+    b.set_groups(groups_mib << 20)
+    b.time_loop(3)
+    b.synchronize()
+    b.close()
+
+
 def counter_means(outdir, counter):
     """Mean value per dispatch of `counter`, by kernel name, from a rocprofv3 database."""
     import glob
@@ -360,6 +378,7 @@ def main():
                     help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
     ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--ternary-probe", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--ensemble-probe", type=int, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -379,6 +398,9 @@ def main():
         return
     if args.ternary_probe:
         ternary_probe(args.ternary_probe)
+        return
+    if args.ensemble_probe is not None:
+        ensemble_probe(args.ensemble_probe)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
