@@ -519,6 +519,9 @@ def main():
         "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "accuracy_fp32_vs_fp64": acc,
         "kernels_ms": {k: round(v, 5) for k, v in r32["kernels"].items()},
+        "kernels_ms_note": (f"HIP events around each launch over {args.profile_steps} steps launched one by one "
+                            "after the timed loop (not graph-replayed, with an event pair per launch); "
+                            "roofline.mean_launch_ms is the timed loop's"),
         "batched_ternary_ensemble": ens,
         "config_accuracy": configs,
     }
