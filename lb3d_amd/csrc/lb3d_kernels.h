@@ -413,7 +413,19 @@ __device__ __forceinline__ void dipole_core(const Phys<R> &P, R hs0, R hs1, R hs
 {
     if constexpr (Dev) rho_s = P.rho0[2] + rho_s;
     R sN = rho_s > R(CLAMP_SURF) ? rho_s : R(CLAMP_SURF);
-    R sd0 = hs0 / sN, sd1 = hs1 / sN, sd2 = hs2 / sN;
+    // FP64 divides as the reference does; the FP32 form multiplies by one
+    // reciprocal per divisor (8 divisions -> 3 per site)
+    R sd0, sd1, sd2;
+    if constexpr (Dev) {
+        const R rs = R(1.0) / sN;
+        sd0 = hs0 * rs;
+        sd1 = hs1 * rs;
+        sd2 = hs2 * rs;
+    } else {
+        sd0 = hs0 / sN;
+        sd1 = hs1 / sN;
+        sd2 = hs2 / sN;
+    }
     R hc0 = R(0), hc1 = R(0), hc2 = R(0);
 #pragma unroll
     for (int i = 0; i < 18; i++) {
@@ -439,10 +451,18 @@ __device__ __forceinline__ void dipole_core(const Phys<R> &P, R hs0, R hs1, R hs
     R e0 = P.third_beta * h0, e1 = P.third_beta * h1, e2 = P.third_beta * h2;
     if (h_m > R(10.0e-3)) {
         R exh = R(1.0) / tanh(h_m);
-        R sd_mag = exh - R(1.0) / h_m;
-        e0 = ((P.beta * h0) / h_m) * sd_mag;
-        e1 = ((P.beta * h1) / h_m) * sd_mag;
-        e2 = ((P.beta * h2) / h_m) * sd_mag;
+        if constexpr (Dev) {
+            const R rh = R(1.0) / h_m;
+            const R sd_mag = exh - rh;
+            e0 = ((P.beta * h0) * rh) * sd_mag;
+            e1 = ((P.beta * h1) * rh) * sd_mag;
+            e2 = ((P.beta * h2) * rh) * sd_mag;
+        } else {
+            R sd_mag = exh - R(1.0) / h_m;
+            e0 = ((P.beta * h0) / h_m) * sd_mag;
+            e1 = ((P.beta * h1) / h_m) * sd_mag;
+            e2 = ((P.beta * h2) / h_m) * sd_mag;
+        }
     }
     e0 = e0 * P.d_0;
     e1 = e1 * P.d_0;
