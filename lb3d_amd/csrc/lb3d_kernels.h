@@ -752,14 +752,35 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
         rt = rt + (P.amass[c] * rhof[c]) * P.omega[c];
     }
     rt = rt > R(CLAMP_RHO) ? rt : R(CLAMP_RHO);
-    R ux = pt0 / rt, uy = pt1 / rt, uz = pt2 / rt;
+    // FP64 divides as the reference does (lb3d_get_sc_u_eq, lb3d_get_sc_du);
+    // the FP32 form multiplies by 1/rt and tau/(amass*rho): 1 + C divisions
+    // per site instead of 3 + 4C
+    R ux, uy, uz;
+    if constexpr (Dev) {
+        const R irt = R(1.0) / rt;
+        ux = pt0 * irt;
+        uy = pt1 * irt;
+        uz = pt2 * irt;
+    } else {
+        ux = pt0 / rt;
+        uy = pt1 / rt;
+        uz = pt2 / rt;
+    }
 #pragma unroll
     for (int c = 0; c < C; c++) {
         R pon = R(1.0) / (rhof[c] > R(CLAMP_RHO) ? rhof[c] : R(CLAMP_RHO));
-        R tp = P.tau[c] * pon;
-        R du0 = (tp * F[c][0]) / P.amass[c];
-        R du1 = (tp * F[c][1]) / P.amass[c];
-        R du2 = (tp * F[c][2]) / P.amass[c];
+        R du0, du1, du2;
+        if constexpr (Dev) {
+            const R k = P.tau_am[c] * pon;
+            du0 = k * F[c][0];
+            du1 = k * F[c][1];
+            du2 = k * F[c][2];
+        } else {
+            R tp = P.tau[c] * pon;
+            du0 = (tp * F[c][0]) / P.amass[c];
+            du1 = (tp * F[c][1]) / P.amass[c];
+            du2 = (tp * F[c][2]) / P.amass[c];
+        }
         R b[19];
         if constexpr (BD == BD_GENERAL) {
             // the other equilibrium forms; Guo's dd = (tau - 1/2) g_accn inside the

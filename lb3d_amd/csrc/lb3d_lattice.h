@@ -39,6 +39,7 @@ struct Phys {
     int body;             // body force box active
     int bmin[3], bmax[3]; // 1-based inclusive box (x, y, z)
     R amass[3], tau[3], omega[3], am_om[3];  // am_om = amass*omega
+    R tau_am[3];                             // tau/amass (the FP32 form's force-to-velocity factor)
     R omega_d, one_m_omega_d, beta, third_beta, d_0;
     R g_br, g_bs, g_ss, two_g_bs, sign_g_bs;
     R g_accn[3];

@@ -31,6 +31,7 @@ Phys<R> make_phys(const lb3d_params &p)
         P.tau[c] = R(p.tau[c]);
         P.omega[c] = R(om);
         P.am_om[c] = R(p.amass[c] * om);
+        P.tau_am[c] = R(p.tau[c] / p.amass[c]);
         P.g_accn[c] = R(p.g_accn[c]);
         double r0 = p.rho0[c] > 0.0 ? p.rho0[c] : 0.7;
         P.rho0[c] = R(r0);
