@@ -38,13 +38,16 @@ def test_binary_256_fp64_mass_and_fp32_against_fp64():
     assert (drift <= 1e-13).all()
     rho64 = g64.density()
     g64.close()
-    two, one = lattice(dims, LB3D_FP32_DEV, **BINARY), lattice(dims, LB3D_FP32_DEV, **BINARY)
+    two, one, lds = (lattice(dims, LB3D_FP32_DEV, **BINARY) for _ in range(3))
     two.set_plan(0)
-    one.set_plan(2, 128, 4, 128, xcd_group=0)  # the benchmark's one-pass plan
-    two.time_loop(100)
-    one.time_loop(100)
+    one.set_plan(2, 256, 2, 128, xcd_group=0)  # the benchmark's one-pass plans: registers
+    lds.set_plan(1, 256, 2, 128, xcd_group=32)  # or an LDS stage
+    for g in (two, one, lds):
+        g.time_loop(100)
     rho_two, rho_one = two.density(), one.density()
     assert np.array_equal(rho_two, rho_one)
+    assert np.array_equal(lds.populations(), one.populations())
+    lds.close()
     err = np.abs(rho_one - rho64).max() / np.abs(rho64).max()
     print(f"FP32 vs FP64 density after 100 steps: Linf {err:.3e}")
     assert err <= 1e-5
