@@ -76,3 +76,37 @@ def test_batch_persistent_equals_sequential_and_single(precision):
         np.testing.assert_array_equal(sb["max_pos"], sg["max_pos"])
         np.testing.assert_array_equal(sb["max_vel"], sg["max_vel"])
         assert sb["fluid_sites"] == np.prod(d)
+
+
+@pytest.mark.parametrize("ncomp,precision", [(3, LB3D_FP32_DEV), (2, LB3D_FP64)])
+def test_batch_mixed_equilibrium_forms(ncomp, precision):
+    """Lattices with bdist 0 and 2 in one batch: each collide launch serves the
+    lattices of its own form (and the grouped plan groups of either form);
+    every lattice equals its single-lattice run bitwise."""
+    forms = [0, 2, 2, 0, 2]
+    per = [dict(lattice_kw(k, ncomp), bdist=forms[k]) for k in range(len(DIMS))]
+
+    def make():
+        b = Batch(DIMS, per_lattice=per, ncomp=ncomp, precision=precision)
+        for k in range(len(DIMS)):
+            b.init_noise(k, 31 + k, amp=(0.01, 0.01, 0.01))
+        return b
+
+    plans = [make(), make(), make()]
+    plans[0].set_plan(0, 64, 1)   # persistent, one group
+    plans[1].set_plan(0, 64, 1)   # persistent, every lattice its own group
+    plans[1].set_groups(1)
+    plans[2].set_plan(1)          # lattice by lattice
+    for b in plans:
+        b.time_loop(7)
+    for k, d in enumerate(DIMS):
+        g = Lattice(*d, ncomp=ncomp, precision=precision, **per[k])
+        g.init_noise(31 + k, amp=(0.01, 0.01, 0.01))
+        g.bc_init()
+        g.time_loop(7)
+        want = g.populations()
+        for j, b in enumerate(plans):
+            np.testing.assert_array_equal(b.populations(k), want, err_msg=f"plan {j} lattice {k} bdist {forms[k]}")
+        g.close()
+    for b in plans:
+        b.close()
