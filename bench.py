@@ -477,6 +477,8 @@ def main():
     ens = None
     if not args.no_ensemble:
         ens = run_ensemble(args.ensemble_steps, 5, args.ensemble_fp64_steps)
+        # the step's algorithmic rate against what a plain copy reaches on this card
+        ens["step_frac_of_copy"] = round(ens["value"] * 1e6 * ens["bytes_per_site"] / 1e9 / ceiling["copy"], 4)
     configs = None
     if not args.no_configs:
         sc = args.config_steps_scale
