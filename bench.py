@@ -317,6 +317,49 @@ def run_ensemble(steps, warmup, seq_steps, nlat=48):
                 speedup_vs_fp64_sequential=round(r["mlups"] / out["fp64_sequential"]["mlups"], 3))
 
 
+def ensemble_accuracy(steps, nlat=48):
+    """The FP32 batched ensemble against the FP64 batched ensemble after `steps`
+    steps from the same initial state: density errors of every lattice (the
+    worst lattice reported), beside the FP64 floor (FP64 against FP64 started
+    from a mean density 2^-51 higher)."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64, Batch
+
+    G = np.array([2.0] * 6 + [1.0] * 13)
+    edges, g_br = ensemble_spec(nlat)
+    dims = [(e, e, e) for e in edges]
+    per = [dict(g_br=g) for g in g_br]
+    common = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
+    rho = {}
+    for tag, prec, shift in (("fp64", LB3D_FP64, 0.0), ("fp32", LB3D_FP32_DEV, 0.0), ("floor", LB3D_FP64, 2.0 ** -51)):
+        b = Batch(dims, per_lattice=per, precision=prec, **common)
+        m = 0.7 + shift
+        for k in range(nlat):
+            b.init_noise(k, 1000 + k, rho_mean=(m, m, m), amp=(0.01, 0.01, 0.0), dipoles=True)
+        b.time_loop(steps)
+        b.synchronize()
+        # densities from the populations (the collision conserves them per site)
+        rho[tag] = [np.einsum("i,cizyx->czyx", G, b.populations(k)) for k in range(nlat)]
+        b.close()
+    out = {}
+    for tag in ("fp32", "floor"):
+        linf = [float(np.abs(a - r).max() / np.abs(r).max()) for a, r in zip(rho[tag], rho["fp64"])]
+        l2 = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(rho[tag], rho["fp64"])]
+        # statistics that survive the flow's amplification of rounding: the
+        # spread of each density field and each component's mass
+        spread = [float(abs(a[0].std() - r[0].std()) / r[0].std()) for a, r in zip(rho[tag], rho["fp64"])]
+        mass = [float(np.max(np.abs(a.sum(axis=(1, 2, 3)) - r.sum(axis=(1, 2, 3))) / r.sum(axis=(1, 2, 3))))
+                for a, r in zip(rho[tag], rho["fp64"])]
+        out[tag] = dict(density_linf_rel=max(linf), density_l2_rel=max(l2),
+                        worst_lattice_edge=edges[int(np.argmax(linf))],
+                        lattices_within_1e5_linf=int(sum(v <= 1e-5 for v in linf)),
+                        density_std_rel=max(spread), mass_rel=max(mass))
+    res = dict(steps=steps, **out["fp32"], fp64_perturbed_floor=out["floor"])
+    res["density_linf_over_floor"] = res["density_linf_rel"] / max(out["floor"]["density_linf_rel"], 1e-300)
+    print(f"[bench] ensemble accuracy after {steps} steps: density Linf {res['density_linf_rel']:.3e} "
+          f"L2 {res['density_l2_rel']:.3e} (floor {out['floor']['density_linf_rel']:.3e})", file=sys.stderr, flush=True)
+    return res
+
+
 def host_threads():
     """Threads for the multi-core CPU baseline: OMP_NUM_THREADS if set, else
     the CPUs this process may run on."""
@@ -370,6 +413,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--ensemble-fp64-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-accuracy-steps", type=int, default=1000,
+                    help="FP32 against FP64 ensemble after this many steps (0: skip)")
     ap.add_argument("--no-floor", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy runs of the ternary and Lees-Edwards configurations")
@@ -479,6 +524,8 @@ def main():
         ens = run_ensemble(args.ensemble_steps, 5, args.ensemble_fp64_steps)
         # the step's algorithmic rate against what a plain copy reaches on this card
         ens["step_frac_of_copy"] = round(ens["value"] * 1e6 * ens["bytes_per_site"] / 1e9 / ceiling["copy"], 4)
+        if args.ensemble_accuracy_steps > 0:
+            ens["accuracy_fp32_vs_fp64"] = ensemble_accuracy(args.ensemble_accuracy_steps)
     configs = None
     if not args.no_configs:
         sc = args.config_steps_scale
