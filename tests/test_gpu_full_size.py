@@ -87,6 +87,16 @@ def test_lees_edwards_256x64x64_mass_and_plans():
     print(f"FP64 Lees-Edwards mass drift after 200 steps: {drift}")
     assert (drift <= 1e-12).all()
     g64.close()
+    # FP64 populations (320 MB) exceed the memory-side cache: the two-pass
+    # collide stores them with the non-temporal hint, the one-pass kernel not
+    a64, b64 = lattice(dims, LB3D_FP64, **kw), lattice(dims, LB3D_FP64, **kw)
+    a64.set_plan(0)
+    b64.set_plan(1, 64, 4, 64, xcd_group=0)
+    a64.time_loop(20)
+    b64.time_loop(20)
+    assert np.array_equal(a64.populations(), b64.populations())
+    a64.close()
+    b64.close()
     two, one = lattice(dims, LB3D_FP32_DEV, **kw), lattice(dims, LB3D_FP32_DEV, **kw)
     two.set_plan(0)
     one.set_plan(1, 64, 8, 64, xcd_group=0)
