@@ -1,0 +1,27 @@
+"""The benchmark's command line without a GPU: one GPU only, and the
+contract's fixed fields."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
+
+
+def test_bytes_per_site_model():
+    """Algorithmic bytes per site update: read and write every population
+    once (and the ternary dipole field), 2*(19*C + 3*ternary)*sizeof(real)."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    assert bench.bytes_per_site(2, False, 4) == 304
+    assert bench.bytes_per_site(2, False, 8) == 608
+    assert bench.bytes_per_site(3, True, 4) == 480
+    assert bench.bytes_per_site(3, True, 8) == 960
