@@ -71,3 +71,13 @@ def test_params_struct_layout():
 def test_sanity_struct_layout():
     # lb3d_sanity: 24 doubles, 27 + 1 int32, 4 int64 (the Fortran bind(C) type mirrors it)
     assert ctypes.sizeof(gpu.Sanity) == 24 * 8 + 28 * 4 + 4 * 8
+
+
+def test_batch_entry_points_reject_a_null_batch():
+    """Plan and group calls on a NULL batch return LB3D_EINVAL (1) and say
+    why, without a device call."""
+    L = gpu.lib()
+    assert L.lb3d_gpu_batch_set_groups(None, 1 << 20) == 1
+    assert b"NULL" in L.lb3d_gpu_last_error()
+    assert L.lb3d_gpu_batch_set_plan(None, 0, 64, 0) == 1
+    assert L.lb3d_gpu_batch_autotune(None, 1, None, 0) == 1
