@@ -267,6 +267,16 @@ int lb3d_gpu_batch_diagnostics(lb3d_batch *b, int32_t k, double mass[3], double 
  * or one lattice after another with the single-lattice kernels
  * (sequential = 1). */
 int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int32_t xcd_group);
+/* Collide-moments plan (FP32 ternary batches): each timestep is the dipole
+ * relaxation, one collision kernel that also gathers the moments of the next
+ * step's streamed state from the populations it has just written, and a
+ * pass over the tile edges whose moments it cannot gather; the populations
+ * are read once per step.  Tiles of about tile_cols columns (tile_cols = 0
+ * turns the plan off, back to set_plan's), z-chunks of about lz planes,
+ * xcd_group work items per XCD chunk.  Results equal the other plans'
+ * bitwise (the FP32 ternary moments are summed in one order by every plan).
+ * Not a reference routine: the reference runs one lattice per job. */
+int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t lz, int32_t xcd_group);
 /* Groups of the persistent plan: consecutive lattices whose populations add
  * up to about group_bytes (a larger lattice alone) run their moments, dipole
  * and collide phases before the next group starts, so that the collide finds
@@ -279,7 +289,8 @@ int lb3d_gpu_batch_set_groups(lb3d_batch *b, uint64_t group_bytes);
  * and lattice groups on the batch's own state (unchanged) and keeps the
  * fastest. */
 int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, int32_t report_len);
-/* Mean HIP-event duration of the moments, dipole and collide phases. */
+/* Mean HIP-event duration of the moments, dipole and collide phases (the
+ * collide-moments plan: its edge moments, dipole and collide launches). */
 int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[3],
                                  double *ms_per_step);
 

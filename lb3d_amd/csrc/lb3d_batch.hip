@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -48,6 +49,15 @@ struct lb3d_batch {
     uint64_t group_bytes = 0;
     std::vector<uint32_t> item0;   // first work item of each lattice, then nitems
     std::vector<uint32_t> groups;  // first work item of each group, then nitems
+    // collide-moments plan (FP32 ternary, lb3d_gpu_batch_set_collide_moments):
+    // tiles of about cm_cols columns and z-chunks of about cm_lz planes
+    int cm_cols = 0;  // 0: the plan is off
+    int cm_lz = 0;
+    void *aux1 = nullptr;  // second aux field: aux(j) holds the moments of f[j]'s streamed state
+    void *cm_items = nullptr, *edge_items = nullptr, *edge_sites = nullptr;
+    uint32_t cm_nitems = 0, edge_nitems = 0;
+    uint64_t cm_edge_sites = 0;  // edge sites (moments pulled by k_batch_edge_moments)
+    bool aux_ok = false;         // aux(cur) holds the moments of the state's streamed populations
 };
 
 namespace {
@@ -59,7 +69,10 @@ DevBufs view(const lb3d_batch *b, int k)
     DevBufs v;
     for (int j = 0; j < 2; j++) v.f[j] = (char *)b->f[j] + b->f_off[k] * b->rs;
     v.aux = (char *)b->aux + b->aux_off[k] * b->rs;
+    // one dipole field per lattice, relaxed in place: the dipole pass writes
+    // db_alt after the moments pass has read db, and the collide reads db_alt
     v.db = b->C == 3 ? (char *)b->db + b->db_off[k] * b->rs : nullptr;
+    v.db_alt = v.db;
     v.cur = b->cur;
     return v;
 }
@@ -92,8 +105,45 @@ void group_phase(lb3d_batch *b, size_t g, int ph)
                 b->C, b->stream);
 }
 
+void *aux_of(const lb3d_batch *b, int j) { return j == 0 ? b->aux : b->aux1; }
+
+// The moments of the state's streamed populations into aux(cur), which the
+// collide-moments plan carries from step to step.
+void prime_aux(lb3d_batch *b)
+{
+    b->B->phase(0, b->lat, b->items, b->nitems, xcd_round(b->nitems), b->f[b->cur], b->f[1 - b->cur],
+                aux_of(b, b->cur), b->db, b->C, b->stream);
+    b->aux_ok = true;
+}
+
+// One step of the collide-moments plan: dipole relaxation from aux(cur),
+// collision with the gathered moments into aux(1-cur), then the moments of
+// the tiles' edge sites.  Phases ph < 0: all three; else only that one
+// (0 edge moments, 1 dipole, 2 collide) without swapping.
+void cm_phase(lb3d_batch *b, int ph)
+{
+    const int c = b->cur;
+    if (ph == 1)
+        b->B->phase(1, b->lat, b->items, b->nitems, xcd_round(b->nitems), b->f[c], b->f[1 - c], aux_of(b, c), b->db,
+                    b->C, b->stream);
+    else if (ph == 2)
+        b->B->cm_collide(b->lat, b->cm_items, b->cm_nitems, xcd_round(b->cm_nitems), b->f[c], b->f[1 - c],
+                         aux_of(b, c), aux_of(b, 1 - c), b->db, b->stream);
+    else
+        b->B->edge_moments(b->lat, b->edge_items, b->edge_nitems, xcd_round(b->edge_nitems), b->edge_sites,
+                           b->f[1 - c], b->db, aux_of(b, 1 - c), b->stream);
+}
+
 void enqueue_batch_step(lb3d_batch *b)
 {
+    if (b->cm_cols) {
+        cm_phase(b, 1);
+        cm_phase(b, 2);
+        cm_phase(b, 0);
+        b->cur = 1 - b->cur;
+        return;
+    }
+    b->aux_ok = false;  // the other plans use aux(0) as scratch
     if (b->sequential) {
         for (int k = 0; k < b->nlat; k++) {
             DevBufs v = view(b, k);
@@ -114,6 +164,92 @@ int drop_graph(lb3d_batch *b)
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    return LB3D_OK;
+}
+
+// Work items and edge-site lists of the collide-moments plan.  Per
+// lattice: kx tiles of tx columns (tx + 2 <= 256, the region's row stride),
+// rows ty = 256 / (tx + 2) evened out over the lattice, z-chunks of about lz
+// planes.  Lattices go largest first, each z-chunk's tiles consecutively (the
+// tiles that share halos run together on one XCD).
+int build_cm(lb3d_batch *b, int cols, int lz)
+{
+    if (!b->B->cm_supported(b->C))
+        return set_error(LB3D_EINVAL, "the collide-moments plan runs FP32 ternary batches");
+    if (cols < 1 || cols > 254 || lz < 1) return set_error(LB3D_EINVAL, "need 1 <= tile_cols <= 254 and lz >= 1");
+    b->cm_cols = 0;
+    b->cm_edge_sites = 0;
+    std::vector<int> order(b->nlat);
+    for (int k = 0; k < b->nlat; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->ns[a] > b->ns[c]; });
+    std::vector<uint4> items;
+    std::vector<uint32_t> sites;
+    std::vector<uint2> eitems;
+    for (int k : order) {
+        const lb3d_params &p = b->p[k];
+        const int nx = p.nx, ny = p.ny, nz = p.nz;
+        if (nx > 65535 || ny > 65535 || nz > 65535) return set_error(LB3D_EINVAL, "lattice edge above 65535");
+        int kx = (nx + cols / 2) / cols;
+        if (kx < 1) kx = 1;
+        int tx = (nx + kx - 1) / kx;
+        while (tx + 2 > 256) tx = (nx + ++kx - 1) / kx;
+        kx = (nx + tx - 1) / tx;
+        const int S = tx + 2;
+        int ty = 256 / S < ny ? 256 / S : ny;
+        int ky = (ny + ty - 1) / ty;
+        ty = (ny + ky - 1) / ky;
+        ky = (ny + ty - 1) / ty;
+        int kz = (nz + lz / 2) / lz;
+        if (kz < 1) kz = 1;
+        int lzc = (nz + kz - 1) / kz;
+        kz = (nz + lzc - 1) / lzc;
+        std::vector<uint32_t> es;
+        for (int zc = 0; zc < kz; zc++)
+            for (int yt = 0; yt < ky; yt++)
+                for (int xt = 0; xt < kx; xt++) {
+                    const int x0 = xt * tx, y0 = yt * ty, z0 = zc * lzc;
+                    const int tw = nx - x0 < tx ? nx - x0 : tx, th = ny - y0 < ty ? ny - y0 : ty;
+                    const int z1 = nz - z0 < lzc ? nz : z0 + lzc;
+                    items.push_back(make_uint4((unsigned)k, (unsigned)x0 | (unsigned)y0 << 16,
+                                               (unsigned)z0 | (unsigned)z1 << 16,
+                                               (unsigned)tw | (unsigned)th << 9 | (unsigned)S << 18));
+                    for (int z = z0; z < z1; z++)
+                        for (int r = 0; r < th; r++) {
+                            const bool edge_row = r == 0 || r == th - 1 || z == z0 || z == z1 - 1;
+                            if (!edge_row) continue;
+                            for (int c = 0; c < tw; c++)
+                                es.push_back((uint32_t)(x0 + c) + (uint32_t)nx * ((uint32_t)(y0 + r) + (uint32_t)ny * z));
+                        }
+                }
+        std::sort(es.begin(), es.end());
+        b->cm_edge_sites += es.size();
+        for (size_t q = 0; q < es.size(); q += BLOCK) eitems.push_back(make_uint2((unsigned)k, (unsigned)(sites.size() + q)));
+        es.resize((es.size() + BLOCK - 1) / BLOCK * BLOCK, ~0u);
+        sites.insert(sites.end(), es.begin(), es.end());
+    }
+    for (void **q : {&b->cm_items, &b->edge_items, &b->edge_sites})
+        if (*q) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+    b->cm_nitems = b->edge_nitems = 0;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+        BTRY(hipMalloc(dst, bytes ? bytes : 16));
+        if (bytes) BTRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return LB3D_OK;
+    };
+    if (int rc = up(&b->cm_items, items.data(), items.size() * sizeof(uint4))) return rc;
+    if (int rc = up(&b->edge_items, eitems.data(), eitems.size() * sizeof(uint2))) return rc;
+    if (int rc = up(&b->edge_sites, sites.data(), sites.size() * sizeof(uint32_t))) return rc;
+    if (!b->aux1) {
+        const uint64_t elems = b->aux_off.back() + aux_elems(b->C, b->ns.back());
+        BTRY(hipMalloc(&b->aux1, elems * b->rs));
+    }
+    b->cm_nitems = (uint32_t)items.size();
+    b->edge_nitems = (uint32_t)eitems.size();
+    b->cm_cols = cols;
+    b->cm_lz = lz;
+    b->aux_ok = false;
     return LB3D_OK;
 }
 
@@ -232,7 +368,8 @@ int lb3d_gpu_batch_destroy(lb3d_batch *b)
     drop_graph(b);
     for (int j = 0; j < 2; j++)
         if (b->f[j]) (void)hipFree(b->f[j]);
-    for (void *q : {b->aux, b->db, b->lat, b->items, (void *)b->partials})
+    for (void *q : {b->aux, b->aux1, b->db, b->lat, b->items, (void *)b->partials, b->cm_items, b->edge_items,
+                    b->edge_sites})
         if (q) (void)hipFree(q);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     for (Launcher *l : b->L) delete l;
@@ -250,6 +387,7 @@ int lb3d_gpu_batch_init_noise(lb3d_batch *b, int32_t k, uint64_t seed, const dou
     BTRY(hipGetLastError());
     BTRY(hipStreamSynchronize(b->stream));
     b->nt = 0;
+    b->aux_ok = false;
     return LB3D_OK;
 }
 
@@ -265,6 +403,7 @@ int lb3d_gpu_batch_upload_sites(lb3d_batch *b, int32_t k, const double *whole_N,
     if (b->C == 3) BTRY(hipMemcpyAsync(v.db, dbb.data(), dbb.size(), hipMemcpyHostToDevice, b->stream));
     BTRY(hipStreamSynchronize(b->stream));
     b->nt = 0;
+    b->aux_ok = false;
     return LB3D_OK;
 }
 
@@ -288,6 +427,7 @@ int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps)
     if (nsteps == 0) return LB3D_OK;
     int rc = ensure_graph(b);
     if (rc) return rc;
+    if (b->cm_cols && !b->aux_ok) prime_aux(b);
     int left = nsteps;
     if (b->cur == 1) {
         enqueue_batch_step(b);
@@ -297,6 +437,7 @@ int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps)
     if (left == 1) enqueue_batch_step(b);
     BTRY(hipGetLastError());
     b->nt += nsteps;
+    b->aux_ok = b->cm_cols != 0;
     return LB3D_OK;
 }
 
@@ -354,7 +495,24 @@ int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int3
     b->xg = (uint32_t)xcd_group;
     b->B->xg = b->xg;
     if (!b->sequential) b->nwg = xcd_round((unsigned)nwg < b->nitems ? (unsigned)nwg : b->nitems);
+    b->cm_cols = 0;
     return drop_graph(b);
+}
+
+int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t lz, int32_t xcd_group)
+{
+    clear_error();
+    if (!b) return set_error(LB3D_EINVAL, "NULL batch");
+    if (xcd_group < 0) return set_error(LB3D_EINVAL, "xcd_group must be >= 0");
+    drop_graph(b);
+    if (tile_cols == 0) {
+        b->cm_cols = 0;
+        return LB3D_OK;
+    }
+    if (int rc = build_cm(b, tile_cols, lz)) return rc;
+    b->xg = (uint32_t)xcd_group;
+    b->B->xg = b->xg;
+    return LB3D_OK;
 }
 
 int lb3d_gpu_batch_set_groups(lb3d_batch *b, uint64_t group_bytes)
@@ -378,12 +536,17 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         bool seq;
         unsigned nwg;
         uint32_t xg;
-        uint64_t gb;  // group bytes of the grouped persistent plan (0: one group)
+        uint64_t gb;     // group bytes of the grouped persistent plan (0: one group)
+        int cols = 0;    // collide-moments plan: tile columns (0: off) and z-chunk
+        int lz = 0;
     };
     std::vector<Cand> cands = {{true, 0, 0, 0}};
     for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
         if (g < b->nitems) cands.push_back({false, g, 0, 0});
     cands.push_back({false, xcd_round(b->nitems), 0, 0});
+    if (b->B->cm_supported(b->C))
+        for (int cols : {14, 16, 24, 30})
+            for (int lz : {16, 32, 64}) cands.push_back({false, 0, 16, 0, cols, lz});
     // the trials overwrite the dipole fields (the dipole pass writes them in
     // place); the guard restores them and the plan if a trial fails, and
     // frees the copy and the events on every path
@@ -393,6 +556,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         unsigned nwg;
         uint32_t xg;
         uint64_t gb;
+        int cols, lz;
         size_t db_bytes = 0;
         void *db_save = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -405,6 +569,11 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
                 b->B->xg = xg;
                 b->group_bytes = gb;
                 make_groups(b);
+                if (cols) {
+                    if (build_cm(b, cols, lz) == LB3D_OK) b->B->xg = xg;
+                } else {
+                    b->cm_cols = 0;
+                }
                 if (db_save) {
                     (void)hipMemcpyAsync(b->db, db_save, db_bytes, hipMemcpyDeviceToDevice, b->stream);
                     (void)hipStreamSynchronize(b->stream);
@@ -414,7 +583,9 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
             if (e1) (void)hipEventDestroy(e1);
             if (db_save) (void)hipFree(db_save);
         }
-    } g{b, b->sequential, b->nwg, b->xg, b->group_bytes};
+    } g{b, b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz};
+    // the trials overwrite aux(0), the scratch of the other plans
+    b->aux_ok = false;
     g.db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
     if (g.db_bytes) {
         BTRY(hipMalloc(&g.db_save, g.db_bytes));
@@ -424,7 +595,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     BTRY(hipEventCreate(&g.e1));
     hipEvent_t e0 = g.e0, e1 = g.e1;
     double best = 1e300;
-    Cand bc = {b->sequential, b->nwg, b->xg, b->group_bytes};
+    Cand bc = {b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz};
     std::string rep;
     auto time_cand = [&](const Cand &c) -> int {
         b->sequential = c.seq;
@@ -432,6 +603,13 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         b->B->xg = c.xg;
         b->group_bytes = c.gb;
         make_groups(b);
+        if (c.cols) {
+            if (c.cols != b->cm_cols || c.lz != b->cm_lz)
+                if (int rc = build_cm(b, c.cols, c.lz)) return rc;
+            prime_aux(b);
+        } else {
+            b->cm_cols = 0;
+        }
         auto trial = [&]() {
             enqueue_batch_step(b);
             b->cur = 1 - b->cur;  // undo the swap: the state stays in f[cur]
@@ -446,9 +624,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         BTRY(hipEventElapsedTime(&ms, e0, e1));
         double per = ms / trial_steps;
         char buf[128];
-        std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
-                      c.seq ? "sequential" : "persistent", c.nwg, c.xg, c.seq ? (size_t)0 : b->groups.size() - 1,
-                      (unsigned long long)(c.gb >> 20), per);
+        if (c.cols)
+            std::snprintf(buf, sizeof buf, "collide-moments cols=%d lz=%d xg=%u %.4f ms\n", c.cols, c.lz, c.xg, per);
+        else
+            std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
+                          c.seq ? "sequential" : "persistent", c.nwg, c.xg, c.seq ? (size_t)0 : b->groups.size() - 1,
+                          (unsigned long long)(c.gb >> 20), per);
         rep += buf;
         if (per < best) {
             best = per;
@@ -458,12 +639,13 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     };
     for (const Cand &c : cands)
         if (int rc = time_cand(c)) return rc;
-    if (!bc.seq) {  // the XCD chunking of the best persistent grid
+    if (!bc.seq) {  // the XCD chunking of the best persistent grid or collide-moments plan
         const Cand top = bc;
         for (uint32_t xg : {1u, 4u, 16u, 64u, 256u})
-            if (int rc = time_cand({false, top.nwg, xg, 0})) return rc;
+            if (xg != top.xg)
+                if (int rc = time_cand({false, top.nwg, xg, 0, top.cols, top.lz})) return rc;
     }
-    if (!bc.seq) {  // then groups of lattices sized for the memory-side cache
+    if (!bc.seq && !bc.cols) {  // then groups of lattices sized for the memory-side cache
         const Cand top = bc;
         uint64_t total = 0;
         for (int k = 0; k < b->nlat; k++) total += (uint64_t)b->C * 19u * b->ns[k] * b->rs;
@@ -484,11 +666,22 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     b->B->xg = bc.xg;
     b->group_bytes = bc.gb;
     make_groups(b);
+    if (bc.cols) {
+        if (bc.cols != b->cm_cols || bc.lz != b->cm_lz)
+            if (int rc = build_cm(b, bc.cols, bc.lz)) return rc;
+    } else {
+        b->cm_cols = 0;
+    }
+    b->aux_ok = false;
     drop_graph(b);
     char buf[128];
-    std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
-                  bc.seq ? "sequential" : "persistent", bc.nwg, bc.xg, bc.seq ? (size_t)0 : b->groups.size() - 1,
-                  (unsigned long long)(bc.gb >> 20), best);
+    if (bc.cols)
+        std::snprintf(buf, sizeof buf, "chosen: collide-moments cols=%d lz=%d xg=%u %.4f ms\n", bc.cols, bc.lz, bc.xg,
+                      best);
+    else
+        std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
+                      bc.seq ? "sequential" : "persistent", bc.nwg, bc.xg, bc.seq ? (size_t)0 : b->groups.size() - 1,
+                      (unsigned long long)(bc.gb >> 20), best);
     rep += buf;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
@@ -505,7 +698,9 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
 {
     clear_error();
     if (!b || nsteps < 1 || !ms_phase || !ms_per_step) return set_error(LB3D_EINVAL, "bad argument");
-    const size_t units = b->sequential ? (size_t)b->nlat : b->groups.size() - 1;
+    const bool cm = b->cm_cols != 0;
+    if (cm && !b->aux_ok) prime_aux(b);
+    const size_t units = cm ? 1 : (b->sequential ? (size_t)b->nlat : b->groups.size() - 1);
     const size_t per = 1 + 3 * units;  // events per step
     // destroyed when the scope ends, also on an early error return
     struct Events {
@@ -523,7 +718,11 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
         BTRY(hipEventRecord(ev[q++], b->stream));
         for (size_t u = 0; u < units; u++) {
             for (int ph = 0; ph < 3; ph++) {
-                if (b->sequential) {
+                if (cm) {
+                    // the plan's order: dipole, collide, edge moments; their
+                    // times go to ms_phase[1], [2] and [0]
+                    cm_phase(b, (ph + 1) % 3);
+                } else if (b->sequential) {
                     DevBufs v = view(b, (int)u);
                     if (ph == 0) b->L[u]->moments(v, b->stream);
                     if (ph == 1) b->L[u]->dipole(v, b->stream);
@@ -543,13 +742,14 @@ int lb3d_gpu_batch_profile_steps(lb3d_batch *b, int32_t nsteps, double ms_phase[
         for (size_t j = 0; j + 1 < per; j++) {
             float m = 0.f;
             BTRY(hipEventElapsedTime(&m, ev[(size_t)st * per + j], ev[(size_t)st * per + j + 1]));
-            tot[j % 3] += m;
+            tot[cm ? (j + 1) % 3 : j % 3] += m;
             all += m;
         }
     }
     for (int ph = 0; ph < 3; ph++) ms_phase[ph] = tot[ph] / nsteps;
     *ms_per_step = all / nsteps;
     b->nt += nsteps;
+    b->aux_ok = cm;
     return LB3D_OK;
 }
 

@@ -93,7 +93,30 @@ struct lb3d_ctx {
     Plan plan;
     bool tuned = false;
     bool aux_from_prev = false;  // aux is recomputable from f[1-cur] (fused binary steps)
+    // ternary dipole fields, tied to the population parity: a step reads
+    // dbk[cur] (the field of the state) and writes dbk[1-cur] (the relaxed
+    // field), so a captured two-step graph finds the live field where it
+    // expects it whatever the host did before
+    void *dbk[2] = {nullptr, nullptr};
 };
+
+// Points b.db / b.db_alt at the dipole fields of the current parity.
+static void set_cur(lb3d_ctx *c, int cur)
+{
+    c->b.cur = cur;
+    c->b.db = c->dbk[cur];
+    c->b.db_alt = c->dbk[1 - cur];
+}
+
+static void flip(lb3d_ctx *c) { set_cur(c, 1 - c->b.cur); }
+
+static void drop_graph(lb3d_ctx *c)
+{
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+}
 
 static int check_params(const lb3d_params *p)
 {
@@ -277,9 +300,10 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
             return cleanup(LB3D_ENOMEM, std::string("hipMalloc populations: ") + hipGetErrorString(e));
     if ((e = hipMalloc(&c->b.aux, c->aux_bytes)) != hipSuccess)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc aux: ") + hipGetErrorString(e));
-    if (c->db_bytes && ((e = hipMalloc(&c->b.db, c->db_bytes)) != hipSuccess ||
-                        (e = hipMalloc(&c->b.db_alt, c->db_bytes)) != hipSuccess))
+    if (c->db_bytes && ((e = hipMalloc(&c->dbk[0], c->db_bytes)) != hipSuccess ||
+                        (e = hipMalloc(&c->dbk[1], c->db_bytes)) != hipSuccess))
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc db: ") + hipGetErrorString(e));
+    set_cur(c, 0);
     if (c->le) {
         if ((e = hipMalloc(&c->b.face_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
             (e = hipMalloc(&c->b.face_aux, 2 * 4 * plane * rs)) != hipSuccess)
@@ -310,12 +334,12 @@ int lb3d_gpu_destroy(lb3d_ctx *c)
 {
     if (!c) return LB3D_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    drop_graph(c);
     for (int k = 0; k < 2; k++)
         if (c->b.f[k]) (void)hipFree(c->b.f[k]);
     if (c->b.aux) (void)hipFree(c->b.aux);
-    if (c->b.db) (void)hipFree(c->b.db);
-    if (c->b.db_alt) (void)hipFree(c->b.db_alt);
+    for (int k = 0; k < 2; k++)
+        if (c->dbk[k]) (void)hipFree(c->dbk[k]);
     if (c->b.face_pc) (void)hipFree(c->b.face_pc);
     if (c->b.face_aux) (void)hipFree(c->b.face_aux);
     if (c->b.le) (void)hipFree(c->b.le);
@@ -350,6 +374,8 @@ static double w_of(int i) { return i < 18 ? 1.0 / 36.0 : 1.0 / 3.0; }
 static int set_walls(lb3d_ctx *c, const uint8_t *mask)
 {
     const size_t ns = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    // a captured graph holds the kernels of the old wall state
+    drop_graph(c);
     if (!mask) {
         c->L->walls = false;
         c->b.rock = nullptr;
@@ -363,10 +389,6 @@ static int set_walls(lb3d_ctx *c, const uint8_t *mask)
     if (c->plan.fused) {
         c->plan = Plan();
         c->L->xg = c->plan.xg;
-    }
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
     }
     return LB3D_OK;
 }
@@ -385,7 +407,7 @@ int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, i
     if (nrock && c->p.inv_fluid == 12)
         return fail(LB3D_EINVAL, "rock sites (walls) and the Zou-He boundary are not supported together");
     if ((rc = set_walls(c, nrock ? rock.data() : nullptr))) return rc;
-    c->b.cur = 0;
+    set_cur(c, 0);
     HIP_TRY(hipMemcpyAsync(c->b.f[0], fb.data(), c->f_bytes, hipMemcpyHostToDevice, c->stream));
     if (c->p.ncomp == 3)
         HIP_TRY(hipMemcpyAsync(c->b.db, dbb.data(), c->db_bytes, hipMemcpyHostToDevice, c->stream));
@@ -405,7 +427,7 @@ int lb3d_gpu_init_noise(lb3d_ctx *c, uint64_t seed, const double rho_mean[3], co
     if (!c || !rho_mean || !amp) return fail(LB3D_EINVAL, "NULL argument");
     int rc = set_walls(c, nullptr);
     if (rc) return rc;
-    c->b.cur = 0;
+    set_cur(c, 0);
     c->L->init_noise(c->b, seed, rho_mean, amp, dipoles, c->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -618,18 +640,20 @@ int lb3d_gpu_advection(lb3d_ctx *c)
     return LB3D_OK;
 }
 
-// commit = false (autotune trials): the ternary one-pass kernel's new dipole
-// field stays in db_alt and db keeps the state
-static void launch_collision(lb3d_ctx *c, bool commit = true)
+// The collision of the current plan: f[cur] -> f[1-cur]; the ternary dipole
+// relaxation reads the state's field db and writes db_alt (both plans).
+// Plans are validated when they are set (lb3d_gpu_set_plan, autotune,
+// create); a launcher that still rejects one returns LB3D_EINVAL.
+static int launch_collision(lb3d_ctx *c)
 {
     if (c->plan.fused) {
-        // plans are validated when set (lb3d_gpu_set_plan, autotune, create)
-        if (!c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream)) abort();
-        if (c->p.ncomp == 3 && commit) std::swap(c->b.db, c->b.db_alt);
+        if (!c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream))
+            return fail(LB3D_EINVAL, "the one-pass plan has no kernel for this lattice");
     } else {
         c->L->dipole(c->b, c->stream);
         c->L->collide(c->b, c->stream);
     }
+    return LB3D_OK;
 }
 
 static void mark_stepped(lb3d_ctx *c)
@@ -643,16 +667,17 @@ int lb3d_gpu_collision(lb3d_ctx *c)
     g_err.clear();
     if (!c) return fail(LB3D_EINVAL, "NULL context");
     if (c->state != AFTER_BC8) return fail(LB3D_ESTATE, "collision must follow bc_apply(8)");
-    launch_collision(c);
+    int rc = launch_collision(c);
+    if (rc) return rc;
     HIP_TRY(hipGetLastError());
-    c->b.cur = 1 - c->b.cur;
+    flip(c);
     c->state = IDLE;
     mark_stepped(c);
     return LB3D_OK;
 }
 
 // One whole timestep on the stream (graph-capturable: kernels only).
-static void enqueue_step(lb3d_ctx *c)
+static int enqueue_step(lb3d_ctx *c)
 {
     if (c->le) {
         launch_le_neighbours(c, -1);
@@ -664,8 +689,10 @@ static void enqueue_step(lb3d_ctx *c)
         c->L->moments(c->b, c->stream);
         if (c->le) c->L->le_faces(c->b, true, c->stream);
     }
-    launch_collision(c);
-    c->b.cur = 1 - c->b.cur;
+    int rc = launch_collision(c);
+    if (rc) return rc;
+    flip(c);
+    return LB3D_OK;
 }
 
 static int ensure_graph(lb3d_ctx *c)
@@ -673,13 +700,17 @@ static int ensure_graph(lb3d_ctx *c)
     if (c->graph) return LB3D_OK;
     // two steps starting from cur = 0 (A -> B -> A)
     int saved = c->b.cur;
-    c->b.cur = 0;
+    set_cur(c, 0);
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c);
-    enqueue_step(c);
+    int rc = enqueue_step(c);
+    if (rc == LB3D_OK) rc = enqueue_step(c);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
-    c->b.cur = saved;
+    set_cur(c, saved);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
     if (e != hipSuccess) return fail(LB3D_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -715,11 +746,11 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     if ((rc = ensure_graph(c))) return rc;
     int left = nsteps;
     if (c->b.cur == 1) {
-        enqueue_step(c);
+        if ((rc = enqueue_step(c))) return rc;
         left--;
     }
     for (; left >= 2; left -= 2) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
-    if (left == 1) enqueue_step(c);
+    if (left == 1 && (rc = enqueue_step(c))) return rc;
     HIP_TRY(hipGetLastError());
     c->nt += nsteps;
     mark_stepped(c);
@@ -796,32 +827,24 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
                     cands.push_back(q);
                 }
     }
-    // the trials overwrite the dipole field (ternary two-pass: the dipole
-    // pass writes it in place); it is restored from a copy afterwards.  The
-    // guard frees the copy on every path and, when a trial fails, also puts
-    // back the field and the plan.
+    // the trials write f[1-cur] and the ternary dipole field db_alt, never
+    // the state (f[cur], db); the guard puts back the plan if a trial fails
     struct Guard {
         lb3d_ctx *c;
         Plan saved;
-        void *db_save = nullptr;
         bool restore = true;
         ~Guard()
         {
             if (restore) {
                 c->plan = saved;
                 c->L->xg = saved.xg;
-                if (db_save) {
-                    (void)hipMemcpyAsync(c->b.db, db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream);
-                    (void)hipStreamSynchronize(c->stream);
-                }
             }
-            if (db_save) (void)hipFree(db_save);
         }
     } g{c, c->plan};
-    if (c->p.ncomp == 3) {
-        HIP_TRY(hipMalloc(&g.db_save, c->db_bytes));
-        HIP_TRY(hipMemcpyAsync(g.db_save, c->b.db, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
-    }
+    // f[1-cur] and db_alt no longer hold the last step's input, so the
+    // moments of the last advection cannot be rebuilt from them afterwards
+    c->aux_valid = false;
+    c->aux_from_prev = false;
     Events ev(2);
     HIP_TRY(ev.create());
     hipEvent_t e0 = ev[0], e1 = ev[1];
@@ -835,11 +858,12 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
             if (c->le) c->L->le_faces(c->b, false, c->stream);
             if (!q.fused) c->L->moments(c->b, c->stream);
             if (c->le) c->L->le_faces(c->b, true, c->stream);
-            launch_collision(c, false);
+            return launch_collision(c);
         };
-        trial();  // warm-up
+        if (int rc = trial()) return rc;  // warm-up
         HIP_TRY(hipEventRecord(e0, c->stream));
-        for (int t = 0; t < trial_steps; t++) trial();
+        for (int t = 0; t < trial_steps; t++)
+            if (int rc = trial()) return rc;
         HIP_TRY(hipEventRecord(e1, c->stream));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e1));
@@ -868,19 +892,11 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         q.xg = xg;
         if (int rc = time_plan(q, per)) return rc;
     }
-    if (g.db_save) {
-        HIP_TRY(hipMemcpyAsync(c->b.db, g.db_save, c->db_bytes, hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
     g.restore = false;
     c->plan = bestp;
     c->L->xg = bestp.xg;
     c->tuned = true;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
-    // aux (ternary two-pass) was overwritten by the trials with values of the same state
+    drop_graph(c);
     char buf[112];
     std::snprintf(buf, sizeof buf, "chosen: %s %dx%d lz=%d %s xg=%u %.4f ms\n", bestp.fused ? "fused" : "two-pass",
                   bestp.fused ? bestp.tx : 0, bestp.fused ? bestp.ty : 0, bestp.fused ? bestp.lz : 0,
@@ -917,10 +933,7 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     }
     c->plan = q;
     c->L->xg = q.xg;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graph(c);
     return LB3D_OK;
 }
 
@@ -1131,7 +1144,8 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
         }
         if (c->plan.fused) {
             HIP_TRY(rec(st, K_FUS, true));
-            c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream);
+            if (!c->L->fused(c->b, c->plan.tx, c->plan.ty, c->plan.lz, c->plan.stage, c->stream))
+                return fail(LB3D_EINVAL, "the one-pass plan has no kernel for this lattice");
             HIP_TRY(rec(st, K_FUS, false));
             mark(K_FUS);
         } else {
@@ -1146,7 +1160,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
             HIP_TRY(rec(st, K_COL, false));
             mark(K_COL);
         }
-        c->b.cur = 1 - c->b.cur;
+        flip(c);
     }
     HIP_TRY(hipEventRecord(t1, c->stream));
     HIP_TRY(hipGetLastError());

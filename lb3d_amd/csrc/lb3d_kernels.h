@@ -37,6 +37,19 @@ struct alignas(4 * sizeof(R)) V4 {
     R a, b, c, d;
 };
 __host__ __device__ constexpr uint64_t aux3_vec_off(uint64_t ns) { return (6 * ns + 3) / 4 * 4; }
+
+// Summation order of the FP32 ternary moments: the directions grouped by
+// their z component, c_z = +1 first, then 0, then -1, ascending within a
+// group.  A kernel that gathers the moments of the next step from the
+// post-collision populations of one z-plane at a time (k_batch_cm) receives
+// them in this order, so every FP32 ternary kernel sums them in it and all
+// plans round alike.  FP64 and binary lattices keep the reference's order.
+__device__ constexpr int ZORD[19] = {4, 8, 12, 14, 16, 0, 1, 2, 3, 6, 7, 10, 11, 18, 5, 9, 13, 15, 17};
+template <bool ZG>
+__device__ __forceinline__ constexpr int dir_at(int ii)
+{
+    return ZG ? ZORD[ii] : ii;
+}
 __host__ __device__ constexpr uint64_t aux_elems(int ncomp, uint64_t ns)
 {
     return ncomp == 3 ? aux3_vec_off(ns) + 4 * ns : (uint64_t)NAUX2 * ns;
@@ -203,6 +216,23 @@ __device__ __forceinline__ R psi_of(R rho, R emr0)
 // populations themselves are not stored: k_collide pulls them again.
 // One term of the advected dipole: the streamed surfactant population v of
 // direction i (stored form) times the dipole field at its source x - c_i.
+// da += ng * db: the FP32 form with an explicit fused multiply-add, so that
+// every kernel rounds it alike whatever the compiler contracts; FP64 as the
+// reference writes it (separate multiply and add).
+template <bool Dev, typename R>
+__device__ __forceinline__ void da_term(R ng, R b0, R b1, R b2, R &da0, R &da1, R &da2)
+{
+    if constexpr (Dev) {
+        da0 = __builtin_fmaf(ng, b0, da0);
+        da1 = __builtin_fmaf(ng, b1, da1);
+        da2 = __builtin_fmaf(ng, b2, da2);
+    } else {
+        da0 = da0 + ng * b0;
+        da1 = da1 + ng * b1;
+        da2 = da2 + ng * b2;
+    }
+}
+
 template <typename R, bool Dev>
 __device__ __forceinline__ void acc_da(const Phys<R> &P, const R *__restrict__ db, int i, R v, int x, int y, int z,
                                        R &da0, R &da1, R &da2)
@@ -211,9 +241,8 @@ __device__ __forceinline__ void acc_da(const Phys<R> &P, const R *__restrict__ d
     R ng = ns_ * R(GD[i]);
     int xs = wrap1(x - CX[i], P.nx), ys = wrap1(y - CY[i], P.ny), zs = wrap1(z - CZ[i], P.nz);
     uint32_t q = sidx(xs, ys, zs, P.nx, P.ny);
-    da0 = da0 + ng * ld_plane(db, q);
-    da1 = da1 + ng * ld_plane(plane_of(db, 1u, P.ns), q);
-    da2 = da2 + ng * ld_plane(plane_of(db, 2u, P.ns), q);
+    da_term<Dev>(ng, ld_plane(db, q), ld_plane(plane_of(db, 1u, P.ns), q), ld_plane(plane_of(db, 2u, P.ns), q),
+                 da0, da1, da2);
 }
 
 // Zou-He flux rule, lbe_invade_zouandheflux (code/lb3d_invasion.F90:2143-2457)
@@ -335,7 +364,8 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
     bool self_rock = false;
     if constexpr (Rock) self_rock = rock[sidx(x, y, z, P.nx, P.ny)] != 0;
 #pragma unroll
-    for (int i = 0; i < 19; i++) {
+    for (int ii = 0; ii < 19; ii++) {
+        const int i = dir_at<Dev && C == 3>(ii);
 #pragma unroll
         for (int c = 0; c < C; c++) {
             R v;
@@ -694,9 +724,17 @@ __device__ __forceinline__ void acc_moments(int i, R v, R &rho, R (&mom)[3])
     if (CZ[i] != 0) mom[2] = mom[2] + ng * R(CZ[i]);
 }
 
-template <typename R, bool Dev, int C, int BD, bool NT = false, typename Get>
+// a relaxation that only stores its populations
+struct NoSink {
+    template <typename R>
+    __device__ __forceinline__ void operator()(int, int, R) const
+    {
+    }
+};
+template <typename R, bool Dev, int C, int BD, bool NT = false, typename Get, typename Sink = NoSink>
 __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
-                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout);
+                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout,
+                                          bool store = true, Sink sink = Sink());
 
 template <typename R, bool Dev, int C, int BD, bool Refetch = false, bool NT = false, typename Get>
 __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][3], int x, int y, int z,
@@ -709,7 +747,8 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
         mom[c][0] = mom[c][1] = mom[c][2] = R(0);
     }
 #pragma unroll
-    for (int i = 0; i < 19; i++) {
+    for (int ii = 0; ii < 19; ii++) {
+        const int i = dir_at<Dev && C == 3>(ii);
 #pragma unroll
         for (int c = 0; c < C; c++) acc_moments<R>(i, get(c, i), rho[c], mom[c]);
     }
@@ -718,10 +757,13 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
 }
 
 // The relaxation of bgk_site_g from the moments (rho, mom) of the streamed
-// populations that get(c, i) returns.
-template <typename R, bool Dev, int C, int BD, bool NT, typename Get>
+// populations that get(c, i) returns.  Each post-collision population also
+// goes to sink(c, i, value) once get(c, i) has been read; store = false skips
+// the stores to fout.
+template <typename R, bool Dev, int C, int BD, bool NT, typename Get, typename Sink>
 __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&rho)[C], const R (&mom)[C][3],
-                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout)
+                                          R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout,
+                                          bool store, Sink sink)
 {
     R rhof[C];  // full densities
 #pragma unroll
@@ -807,7 +849,9 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
                 R wi = i < 18 ? R(1.0 / 36.0) : R(1.0 / 3.0);
                 R deq = wi * rho[c] + rhof[c] * b[i];
                 R v = get(c, i);
-                st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - deq));
+                R nv = v - P.omega[c] * (v - deq);
+                if (store) st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, nv);
+                sink(c, i, nv);
             }
         } else {
             if constexpr (BD != BD_GENERAL) boltz_dist<R>(BD, ux, uy, uz, du0, du1, du2, b);
@@ -815,7 +859,9 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
             for (int i = 0; i < 19; i++) {
                 R neq = rho[c] * b[i];
                 R v = get(c, i);
-                st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, v - P.omega[c] * (v - neq));
+                R nv = v - P.omega[c] * (v - neq);
+                if (store) st_pop<NT>(fout + (size_t)((uint32_t)c * 19u + (uint32_t)i) * P.ns, s, nv);
+                sink(c, i, nv);
             }
         }
     }
@@ -1220,7 +1266,8 @@ __global__ __launch_bounds__(TX *TY) void k_fused3(
                 hm[c][0] = hm[c][1] = hm[c][2] = R(0);
             }
 #pragma unroll
-            for (int i = 0; i < 19; i++) {
+            for (int ii = 0; ii < 19; ii++) {
+                const int i = dir_at<Dev>(ii);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     R v = pull_flat<R, 3>(f, P, c, i, xo, yo, zw);
@@ -1607,6 +1654,224 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         if (s < L.P.ns && L.P.bdist == BD)
             collide_site<R, Dev, C, false, BD, false, false, NT ? 2 : 0>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
                                                                C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
+    }
+}
+
+// --------------------------------------------------------------------------
+// The collide-moments plan of the batched ternary ensemble.  A step is three
+// launches: k_batch_dipole (the relaxed dipole field, from the moments aux_in
+// of the step's streamed state), k_batch_cm (the collision, which also
+// gathers the moments of the next step's streamed state from the
+// post-collision populations it has just computed) and
+// k_batch_edge_moments (the moments of the sites that k_batch_cm cannot
+// gather).  The populations are read once per step (the three-pass plan
+// reads them twice: once for the moments, once for the collision), and the
+// moments of the streamed state are carried from one step to the next in a
+// second aux field.
+//
+// Work item of k_batch_cm (uint4): x = lattice, y = x0 | y0 << 16,
+// z = z0 | z1 << 16, w = tw | th << 9 | S << 18: a tile of tw columns and th
+// rows of the lattice from (x0, y0), planes z0 .. z1-1, S >= tw + 2 the row
+// stride of the work-group's region (S * th <= 256).
+// Thread t holds region column rx = t % S, row ry = t / S, lattice site
+// (x0 - 1 + rx (wrapped), y0 + ry).  Columns 1..tw are the tile's own;
+// columns 0 and tw + 1 belong to the x neighbours and are collided again
+// here (not stored), so that the post-collision populations that stream into
+// the own columns are all at hand.  For each plane k = z0 .. z1-1:
+//   1. collide the site of plane k (Shan-Chen forces from aux_in and the
+//      relaxed dipole field db, pull, BGK; code/lb3d_collision.F90:122-289);
+//      own columns store the populations to fout;
+//   2. the post-collision populations of the 16 directions with an in-plane
+//      component, and the site's dipole field, go to LDS;
+//   3. each own site of rows 1..th-2 gathers what streams into it from plane
+//      k (code/lb3d_advection.F90:54-151): directions with c_z = +1 into its
+//      site of plane k+1, 0 into plane k, -1 into plane k-1, each group in
+//      ascending order (ZORD), accumulating the density of each component and
+//      the advected dipole da = sum_i g_i n_s,i db(x - c_i);
+//   4. plane k-1 has then received all 19 directions; for planes z0+1 ..
+//      z1-2 its moments go to aux_out in the layout of k_moments.
+// Rows 0 and th-1 and planes z0 and z1-1 of each tile are its edge sites:
+// some of their sources lie in other work-groups' tiles, and
+// k_batch_edge_moments pulls their moments from fout afterwards.
+constexpr int CM_NXY = 16;  // directions with an in-plane component
+// LDS slot of direction i (-1: c_x = c_y = 0, the site's own populations)
+__device__ constexpr int CM_SLOT[19] = {0, 1, 2, 3, -1, -1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, -1};
+
+__device__ __forceinline__ uint4 cm_item(const uint4 *__restrict__ items, uint32_t it)
+{
+    uint4 w = items[it];
+    w.x = __builtin_amdgcn_readfirstlane(w.x);
+    w.y = __builtin_amdgcn_readfirstlane(w.y);
+    w.z = __builtin_amdgcn_readfirstlane(w.z);
+    w.w = __builtin_amdgcn_readfirstlane(w.w);
+    return w;
+}
+
+// The moments of one site (psi, rho per component, the dipole-relaxation
+// 4-vector) to the aux field, as moments_site writes them.
+template <typename R, bool Dev>
+__device__ __forceinline__ void put_aux3(const Phys<R> &P, R *__restrict__ aux, uint32_t s, const R (&rho)[3],
+                                         R da0, R da1, R da2)
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
+        st_plane(aux + (size_t)(3 + c) * P.ns, s, rho[c]);
+    }
+    V4<R> *vec = (V4<R> *)(aux + aux3_vec_off(P.ns));
+    vec[s] = V4<R>{rho[1] - rho[0], da0, da1, da2};
+}
+
+template <typename R, bool Dev, int BD>
+__global__ __launch_bounds__(256) void k_batch_cm(const BatchLat<R> *__restrict__ lat,
+                                                  const uint4 *__restrict__ items, uint32_t nitems,
+                                                  const R *__restrict__ f, const R *__restrict__ aux,
+                                                  const R *__restrict__ db, R *__restrict__ fout,
+                                                  R *__restrict__ aux_out, uint32_t xg)
+{
+    static_assert(Dev, "the collide-moments plan runs the FP32 deviation form (ZORD order)");
+    __shared__ R xb[3][CM_NXY][256];  // post-collision populations of plane k
+    __shared__ R dbs[3][256];         // dipole field of plane k
+    const int tid = threadIdx.x;
+    const uint32_t n8 = xcd_round(nitems);
+    for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
+        const uint32_t it = xcd_tile(t, n8, xg);
+        if (it >= nitems) continue;  // uniform over the block
+        const uint4 w = cm_item(items, it);
+        const BatchLat<R> &L = lat[w.x];
+        if (L.P.bdist != BD) continue;  // uniform: this launch runs one equilibrium form
+        const Phys<R> &P = L.P;
+        const R *fl = f + L.f_off;
+        R *fo = fout + L.f_off;
+        const R *al = aux + L.aux_off;
+        R *ao = aux_out + L.aux_off;
+        const R *dl = db + L.db_off;
+        const int x0 = (int)(w.y & 0xffffu), y0 = (int)(w.y >> 16);
+        const int z0 = (int)(w.z & 0xffffu), z1 = (int)(w.z >> 16);
+        const int tw = (int)(w.w & 511u), th = (int)((w.w >> 9) & 511u), S = (int)(w.w >> 18);
+        const int rx = tid % S, ry = tid / S;
+        const bool act = ry < th && rx <= tw + 1;
+        const bool own = act && rx >= 1 && rx <= tw;
+        const bool gat = own && ry >= 1 && ry <= th - 2;
+        const int x = wrap1(x0 - 1 + rx, P.nx), y = act ? y0 + ry : 0;
+        // densities (component, plane k-1 / k / k+1) and advected dipole
+        // (plane, axis) of the own site, accumulated over the source planes
+        R ar[3][3], ad[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) ar[a][b] = ad[a][b] = R(0);
+        for (int k = z0; k < z1; k++) {
+            // the site's own populations of the directions without an
+            // in-plane component (4, 5 and the rest direction 18)
+            R nz[3][3];
+            R dbx[3] = {R(0), R(0), R(0)};
+            __syncthreads();  // every gather of plane k-1 is done with xb and dbs
+            if (act) {
+                // opaque copies of the lane coordinates: keeps the compiler
+                // from hoisting the per-direction neighbour offsets out of
+                // the z loop (dozens of registers held across the march)
+                int xo = x, yo = y;
+                asm volatile("" : "+v"(xo), "+v"(yo));
+                const uint32_t s = sidx(xo, yo, k, P.nx, P.ny);
+                R F[3][3];
+                sc_forces<R, Dev, 3, false>(P, al, nullptr, dl, xo, yo, k, s, F);
+                asm volatile("" ::: "memory");
+                R n[3][19];
+#pragma unroll
+                for (int i = 0; i < 19; i++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) n[c][i] = pull<R, 3, false>(fl, nullptr, P, c, i, xo, yo, k);
+                R rho[3], mom[3][3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    rho[c] = R(0);
+                    mom[c][0] = mom[c][1] = mom[c][2] = R(0);
+                }
+#pragma unroll
+                for (int ii = 0; ii < 19; ii++) {
+                    const int i = ZORD[ii];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) acc_moments<R>(i, n[c][i], rho[c], mom[c]);
+                }
+                bgk_relax<R, Dev, 3, BD, false>(
+                    P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, rho, mom, F, xo, yo, k,
+                    s, fo, own, [&](int c, int i, R nv) __attribute__((always_inline)) {
+                        if (CM_SLOT[i] >= 0)
+                            xb[c][CM_SLOT[i]][tid] = nv;
+                        else
+                            nz[c][i == 4 ? 0 : (i == 5 ? 1 : 2)] = nv;
+                    });
+                dbx[0] = ld_plane(dl, s);
+                dbx[1] = ld_plane(plane_of(dl, 1u, P.ns), s);
+                dbx[2] = ld_plane(plane_of(dl, 2u, P.ns), s);
+#pragma unroll
+                for (int a = 0; a < 3; a++) dbs[a][tid] = dbx[a];
+            }
+            __syncthreads();
+            if (gat) {
+                // an opaque copy of the thread index: the 16 source slots
+                // are computed here, not hoisted out of the z loop
+                int tq = tid, sq = S;
+                asm volatile("" : "+v"(tq), "+s"(sq));
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+#pragma unroll
+                    for (int ii = 0; ii < 19; ii++) {
+                        const int i = ZORD[ii];
+                        const int q = tq - CX[i] - CY[i] * sq;  // source region slot
+                        const R v = CM_SLOT[i] < 0 ? nz[c][i == 4 ? 0 : (i == 5 ? 1 : 2)]
+                                                   : xb[c][CM_SLOT[i] < 0 ? 0 : CM_SLOT[i]][q];
+                        const int d = CZ[i] + 1;  // destination plane k - 1, k or k + 1
+                        ar[c][d] = ar[c][d] + v * R(GD[i]);
+                        if (c == 2) {
+                            const R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
+                            const R ng = ns_ * R(GD[i]);
+                            const R b0 = CM_SLOT[i] < 0 ? dbx[0] : dbs[0][q];
+                            const R b1 = CM_SLOT[i] < 0 ? dbx[1] : dbs[1][q];
+                            const R b2 = CM_SLOT[i] < 0 ? dbx[2] : dbs[2][q];
+                            da_term<Dev>(ng, b0, b1, b2, ad[d][0], ad[d][1], ad[d][2]);
+                        }
+                    }
+                }
+                // plane k-1 has received all its directions
+                if (k - 1 >= z0 + 1 && k - 1 <= z1 - 2) {
+                    const R rho[3] = {ar[0][0], ar[1][0], ar[2][0]};
+                    put_aux3<R, Dev>(P, ao, sidx(x, y, k - 1, P.nx, P.ny), rho, ad[0][0], ad[0][1], ad[0][2]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                ar[a][0] = ar[a][1];
+                ar[a][1] = ar[a][2];
+                ar[a][2] = R(0);
+                ad[0][a] = ad[1][a];
+                ad[1][a] = ad[2][a];
+                ad[2][a] = R(0);
+            }
+        }
+    }
+}
+
+// The moments of the edge sites of k_batch_cm's tiles (the aux layout of
+// k_moments), pulled from the post-collision populations f.  Work item:
+// (lattice, first entry of its run of 256 site indices); ~0u entries pad a
+// lattice's list to whole runs.
+template <typename R, bool Dev>
+__global__ __launch_bounds__(256) void k_batch_edge_moments(const BatchLat<R> *__restrict__ lat,
+                                                            const uint2 *__restrict__ items, uint32_t nitems,
+                                                            const uint32_t *__restrict__ sites,
+                                                            const R *__restrict__ f, const R *__restrict__ db,
+                                                            R *__restrict__ aux, uint32_t xg)
+{
+    const uint32_t n8 = xcd_round(nitems);
+    for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
+        uint32_t it = xcd_tile(t, n8, xg);
+        if (it >= nitems) continue;
+        uint2 w = batch_item(items, it);
+        const BatchLat<R> &L = lat[w.x];
+        const uint32_t s = sites[w.y + threadIdx.x];
+        if (s < L.P.ns) moments_site<R, Dev, 3, false>(L.P, f + L.f_off, nullptr, db + L.db_off, aux + L.aux_off, s);
     }
 }
 

@@ -275,7 +275,8 @@ class LauncherT : public Launcher {
     }
     void dipole(const DevBufs &b, hipStream_t s) override
     {
-        if (C == 3) k_dipole<R, Dev><<<tiles(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db, xg);
+        // the relaxed field goes to db_alt (the collide reads it there)
+        if (C == 3) k_dipole<R, Dev><<<tiles(), BLOCK, 0, s>>>(P, (const R *)b.aux, (R *)b.db_alt, xg);
     }
     template <int BD, bool NT>
     void collide_bd(const DevBufs &b, hipStream_t s)
@@ -284,13 +285,13 @@ class LauncherT : public Launcher {
         R *fo = (R *)b.f[1 - b.cur];
         if (P.zh && C == 3)
             k_collide<R, Dev, 3, false, BD, false, true, NT><<<tiles(), BLOCK, 0, s>>>(
-                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db, fo, xg, nullptr);
+                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db_alt, fo, xg, nullptr);
         else if (P.zh && !le && !b.rock)
             k_collide<R, Dev, 2, false, BD, false, true, NT><<<tiles(), BLOCK, 0, s>>>(
                 P, f, nullptr, (const R *)b.aux, nullptr, nullptr, fo, xg, nullptr);
         else if (C == 3)
             k_collide<R, Dev, 3, false, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
-                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db, fo, xg, nullptr);
+                P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db_alt, fo, xg, nullptr);
         else if (le)
             k_collide<R, Dev, 2, true, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
                 P, f, (const R *)b.face_pc, (const R *)b.aux, (const R *)b.face_aux, nullptr, fo, xg, nullptr);
@@ -503,6 +504,31 @@ class BatchLauncherT : public BatchLauncher {
                     collide_bd<false>(bd, L, it, nitems, nwg, f, fout, aux, db, ncomp, s);
             }
         }
+    }
+    bool cm_supported(int ncomp) const override { return Dev && ncomp == 3; }
+    void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,
+                    const void *aux_in, void *aux_out, const void *db, hipStream_t s) override
+    {
+        if constexpr (Dev) {
+            const BatchLat<R> *L = (const BatchLat<R> *)lat;
+            const uint4 *it = (const uint4 *)items;
+            // one launch per equilibrium form present; a launch skips the
+            // tiles of the other form
+            if (bd_mask & 1)
+                k_batch_cm<R, Dev, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux_in,
+                                                            (const R *)db, (R *)fout, (R *)aux_out, xg);
+            if (bd_mask & 4)
+                k_batch_cm<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux_in,
+                                                            (const R *)db, (R *)fout, (R *)aux_out, xg);
+        }
+    }
+    void edge_moments(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *sites,
+                      const void *f, const void *db, void *aux, hipStream_t s) override
+    {
+        if constexpr (Dev)
+            k_batch_edge_moments<R, Dev><<<nwg, BLOCK, 0, s>>>((const BatchLat<R> *)lat, (const uint2 *)items,
+                                                               nitems, (const uint32_t *)sites, (const R *)f,
+                                                               (const R *)db, (R *)aux, xg);
     }
     template <bool NT>
     void collide_bd(int bd, const BatchLat<R> *L, const uint2 *it, uint32_t nitems, unsigned nwg, const void *f,

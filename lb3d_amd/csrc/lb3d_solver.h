@@ -15,8 +15,13 @@ namespace lb3d {
 // of step nt; collide writes f[1-cur].
 struct DevBufs {
     void *f[2] = {nullptr, nullptr};
+    // ternary: db is the dipole field of the state (read by the moments and
+    // the one-pass kernel), db_alt receives the relaxed field of the step
+    // (written by the dipole pass or the one-pass kernel, read by the collide);
+    // a single lattice alternates the two with the population parity, the
+    // batch relaxes one field in place (db_alt == db)
     void *aux = nullptr, *db = nullptr, *face_pc = nullptr, *face_aux = nullptr;
-    void *db_alt = nullptr;  // ternary one-pass plan: the new dipole field, swapped with db per step
+    void *db_alt = nullptr;
     const uint8_t *rock = nullptr;  // 1 at rock sites, or nullptr for a lattice without walls
     LEState *le = nullptr;
     int cur = 0;
@@ -79,6 +84,14 @@ class BatchLauncher {
     // phase ph (0 moments, 1 dipole, 2 collide) over nitems work items
     virtual void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg,
                        const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
+    // the collide-moments plan (FP32 ternary only): k_batch_cm over nitems
+    // tiles (uint4 items), and k_batch_edge_moments over nitems runs of 256
+    // edge sites (uint2 items into the site list)
+    virtual bool cm_supported(int ncomp) const = 0;
+    virtual void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
+                            void *fout, const void *aux_in, void *aux_out, const void *db, hipStream_t s) = 0;
+    virtual void edge_moments(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *sites,
+                              const void *f, const void *db, void *aux, hipStream_t s) = 0;
 };
 
 Launcher *make_launcher_fp64(const lb3d_params &p);

@@ -41,7 +41,7 @@ EXPORTS = [
     "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
     "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_set_groups", "lb3d_gpu_batch_autotune",
-    "lb3d_gpu_batch_profile_steps",
+    "lb3d_gpu_batch_profile_steps", "lb3d_gpu_batch_set_collide_moments",
 ]
 
 
@@ -139,6 +139,7 @@ def lib():
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
         L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip, ip]
+        L.lb3d_gpu_batch_set_collide_moments.argtypes = [vp, ip, ip, ip]
         L.lb3d_gpu_batch_set_groups.argtypes = [vp, ctypes.c_uint64]
         L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
@@ -460,6 +461,13 @@ class Batch:
     def set_plan(self, sequential, nwg=2048, xcd_group=0):
         check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg), int(xcd_group)),
               "batch_set_plan")
+
+    def set_collide_moments(self, tile_cols=16, lz=32, xcd_group=16):
+        """The collide-moments plan (FP32 ternary): the collision gathers the
+        next step's moments, populations are read once per step; tile_cols = 0
+        turns it off."""
+        check(lib().lb3d_gpu_batch_set_collide_moments(self._h, int(tile_cols), int(lz), int(xcd_group)),
+              "batch_set_collide_moments")
 
     def set_groups(self, group_bytes):
         """Run the persistent plan in groups of lattices of about group_bytes

@@ -110,3 +110,66 @@ def test_batch_mixed_equilibrium_forms(ncomp, precision):
         g.close()
     for b in plans:
         b.close()
+
+
+CM_DIMS = [(20, 13, 11), (7, 5, 3), (33, 9, 6), (24, 24, 24), (16, 12, 9), (2, 3, 2)]
+
+
+def cm_batch(forms, seed0=50):
+    per = [dict(lattice_kw(k, 3), bdist=forms[k % len(forms)]) for k in range(len(CM_DIMS))]
+    b = Batch(CM_DIMS, per_lattice=per, ncomp=3, precision=LB3D_FP32_DEV)
+    for k in range(len(CM_DIMS)):
+        b.init_noise(k, seed0 + k, amp=(0.02, 0.02, 0.01))
+    return b, per
+
+
+@pytest.mark.parametrize("cols,lz", [(16, 32), (14, 1), (30, 2), (6, 5), (254, 64)])
+def test_collide_moments_plan_equals_three_pass_bitwise(cols, lz):
+    """The collide-moments plan (moments gathered from the post-collision
+    populations, edge sites pulled) against the persistent three-pass plan
+    and single-lattice runs: lattices that do not divide into tiles, z-chunks
+    of 1 and 2 planes (every site an edge), both equilibrium forms, odd step
+    counts around the captured two-step graph."""
+    forms = [2, 0, 2, 2, 0, 2]
+    ref, per = cm_batch(forms)
+    ref.set_plan(0, 64, 1)
+    ref.time_loop(7)
+    b, _ = cm_batch(forms)
+    b.set_collide_moments(cols, lz, 4)
+    b.time_loop(3)
+    b.time_loop(4)
+    assert b.nt == 7
+    for k, d in enumerate(CM_DIMS):
+        np.testing.assert_array_equal(b.populations(k), ref.populations(k), err_msg=f"lattice {k} {d}")
+    g = Lattice(*CM_DIMS[3], ncomp=3, precision=LB3D_FP32_DEV, **per[3])
+    g.init_noise(53, amp=(0.02, 0.02, 0.01))
+    g.bc_init()
+    g.time_loop(7)
+    np.testing.assert_array_equal(b.populations(3), g.populations())
+
+
+def test_collide_moments_plan_switches_and_reinit():
+    """The moments carried between steps are rebuilt whenever they may be
+    stale: after another plan's steps, a re-initialised lattice, the tuner."""
+    forms = [2]
+    ref, _ = cm_batch(forms)
+    ref.set_plan(0, 64, 1)
+    ref.time_loop(3)
+    ref.init_noise(2, 77, amp=(0.02, 0.02, 0.01))
+    ref.time_loop(6)
+    b, _ = cm_batch(forms)
+    b.set_collide_moments(16, 8, 16)
+    b.time_loop(2)
+    b.set_plan(0, 64, 1)
+    b.time_loop(1)
+    b.init_noise(2, 77, amp=(0.02, 0.02, 0.01))
+    b.set_collide_moments(24, 4, 1)
+    b.time_loop(3)
+    b.autotune(1)
+    b.set_collide_moments(16, 32, 16)
+    ms, per_step = b.profile_steps(1)
+    assert per_step > 0
+    b.time_loop(2)
+    assert b.nt == 6  # init_noise restarts the batch's step count
+    for k in range(len(CM_DIMS)):
+        np.testing.assert_array_equal(b.populations(k), ref.populations(k), err_msg=f"lattice {k}")

@@ -293,3 +293,26 @@ def test_fp32_deviation_ternary_and_le():
         rel = np.abs(r32 - r64).max() / np.abs(r64).max()
         print(f"fp32 {kw} x100 Linf rel rho error {rel:.3e}")
         assert rel < 1e-5
+
+
+def test_walls_graph_dropped_when_walls_go():
+    """A lattice that ran with walls, then re-initialised without them, steps
+    like a fresh lattice (the captured graph of the wall kernels is dropped)."""
+    from tests.test_oracle_golden import CASES
+    gold = np.load(GOLD)
+    kw = dict(CASES["wettingWall"])
+    kw.pop("obs")
+    kw.pop("lees_edwards")
+    o = Oracle(16, 16, 16, **kw)
+    o.load_checkpoint_array(gold["wettingWall/checkpoint"])
+    o.init_rock(gold["wettingWall/wall"], 0.0)
+    g = Lattice(16, 16, 16, **kw)
+    g.upload_sites(o.N)
+    g.bc_init()
+    g.time_loop(2)
+    g.init_noise(7)
+    g.time_loop(4)
+    fresh = Lattice(16, 16, 16, **kw)
+    fresh.init_noise(7)
+    fresh.time_loop(4)
+    assert np.array_equal(g.populations(), fresh.populations())

@@ -177,3 +177,62 @@ def test_ternary_one_pass_fp64_matches_oracle_and_downloads():
     I = o.interior()
     for off in (19, 43, 67, 72, 73, 74, 75, 76, 77):
         np.testing.assert_allclose(N[1:-1, 1:-1, 1:-1, off], I[..., off], rtol=0, atol=1e-13, err_msg=str(off))
+
+
+def test_ternary_one_pass_graph_after_staged_step_and_reinit():
+    """The dipole field follows the population parity: a graph captured after
+    an odd number of staged steps, a re-initialised state and the per-kernel
+    profiling all leave the ternary one-pass plan equal to the three-pass plan."""
+    kw = dict(TERN_KW, precision=LB3D_FP32_DEV)
+    dims = (20, 13, 11)
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0)
+    ref.time_loop(4)
+    want = ref.populations()
+    want_db = ref.download_sites()[1:-1, 1:-1, 1:-1, 75:78]
+
+    _, g = pair(*dims, **kw)
+    g.set_plan(3, 32, 4, 4)
+    g.time_step()  # odd parity before the graph is captured
+    g.time_loop(3)
+    assert np.array_equal(g.populations(), want)
+    assert np.array_equal(g.download_sites()[1:-1, 1:-1, 1:-1, 75:78], want_db)
+
+    # profile_steps advances the state like time_loop
+    _, p = pair(*dims, **kw)
+    p.set_plan(3, 32, 4, 4)
+    p.time_step()
+    p.profile_steps(3)
+    assert np.array_equal(p.populations(), want)
+    assert np.array_equal(p.download_sites()[1:-1, 1:-1, 1:-1, 75:78], want_db)
+
+    # init after an odd step count, with a graph already captured
+    _, h = pair(*dims, **kw)
+    h.set_plan(3, 32, 4, 4)
+    h.time_loop(3)
+    h.init_noise(5, amp=(0.02, 0.02, 0.01))
+    h.time_loop(2)
+    fresh = Lattice(*dims, **{k: v for k, v in kw.items() if k != "amp"})
+    fresh.set_plan(3, 32, 4, 4)
+    fresh.init_noise(5, amp=(0.02, 0.02, 0.01))
+    fresh.time_loop(2)
+    assert np.array_equal(h.populations(), fresh.populations())
+    assert np.array_equal(h.download_sites()[1:-1, 1:-1, 1:-1, 75:78],
+                          fresh.download_sites()[1:-1, 1:-1, 1:-1, 75:78])
+
+
+def test_autotune_keeps_state_and_plan_runs_after_it():
+    """The tuner's trials write f[1-cur] and the spare dipole field only."""
+    kw = dict(TERN_KW, precision=LB3D_FP32_DEV)
+    dims = (24, 16, 12)
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0)
+    ref.time_loop(5)
+    want = ref.populations()
+    _, g = pair(*dims, **kw)
+    g.set_plan(3, 32, 8, 4)
+    g.time_loop(3)
+    g.autotune(1)
+    g.set_plan(0)
+    g.time_loop(2)
+    assert np.array_equal(g.populations(), want)
