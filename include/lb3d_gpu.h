@@ -271,12 +271,14 @@ int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int3
  * relaxation, one collision kernel that also gathers the moments of the next
  * step's streamed state from the populations it has just written, and a
  * pass over the tile edges whose moments it cannot gather; the populations
- * are read once per step.  Tiles of about tile_cols columns (tile_cols = 0
- * turns the plan off, back to set_plan's), z-chunks of about lz planes,
+ * are read once per step.  Tiles of about tile_cols columns and as many
+ * rows as work-groups of tile_threads (256 or 512) hold (tile_cols = 0 turns
+ * the plan off, back to set_plan's), z-chunks of about lz planes,
  * xcd_group work items per XCD chunk.  Results equal the other plans'
  * bitwise (the FP32 ternary moments are summed in one order by every plan).
  * Not a reference routine: the reference runs one lattice per job. */
-int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t lz, int32_t xcd_group);
+int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t tile_threads, int32_t lz,
+                                       int32_t xcd_group);
 /* Groups of the persistent plan: consecutive lattices whose populations add
  * up to about group_bytes (a larger lattice alone) run their moments, dipole
  * and collide phases before the next group starts, so that the collide finds

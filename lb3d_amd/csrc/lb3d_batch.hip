@@ -53,6 +53,7 @@ struct lb3d_batch {
     // tiles of about cm_cols columns and z-chunks of about cm_lz planes
     int cm_cols = 0;  // 0: the plan is off
     int cm_lz = 0;
+    int cm_threads = 256;  // work-group size of k_batch_cm (256 or 512)
     void *aux1 = nullptr;  // second aux field: aux(j) holds the moments of f[j]'s streamed state
     void *cm_items = nullptr, *edge_items = nullptr, *edge_sites = nullptr;
     uint32_t cm_nitems = 0, edge_nitems = 0;
@@ -127,8 +128,8 @@ void cm_phase(lb3d_batch *b, int ph)
         b->B->phase(1, b->lat, b->items, b->nitems, xcd_round(b->nitems), b->f[c], b->f[1 - c], aux_of(b, c), b->db,
                     b->C, b->stream);
     else if (ph == 2)
-        b->B->cm_collide(b->lat, b->cm_items, b->cm_nitems, xcd_round(b->cm_nitems), b->f[c], b->f[1 - c],
-                         aux_of(b, c), aux_of(b, 1 - c), b->db, b->stream);
+        b->B->cm_collide(b->lat, b->cm_items, b->cm_nitems, xcd_round(b->cm_nitems), b->cm_threads, b->f[c],
+                         b->f[1 - c], aux_of(b, c), aux_of(b, 1 - c), b->db, b->stream);
     else
         b->B->edge_moments(b->lat, b->edge_items, b->edge_nitems, xcd_round(b->edge_nitems), b->edge_sites,
                            b->f[1 - c], b->db, aux_of(b, 1 - c), b->stream);
@@ -168,15 +169,17 @@ int drop_graph(lb3d_batch *b)
 }
 
 // Work items and edge-site lists of the collide-moments plan.  Per
-// lattice: kx tiles of tx columns (tx + 2 <= 256, the region's row stride),
-// rows ty = 256 / (tx + 2) evened out over the lattice, z-chunks of about lz
-// planes.  Lattices go largest first, each z-chunk's tiles consecutively (the
-// tiles that share halos run together on one XCD).
-int build_cm(lb3d_batch *b, int cols, int lz)
+// lattice: kx tiles of tx columns (tx + 2 <= threads, the region's row
+// stride), rows ty = threads / (tx + 2) evened out over the lattice,
+// z-chunks of about lz planes.  Lattices go largest first, each z-chunk's
+// tiles consecutively (the tiles that share halos run together on one XCD).
+int build_cm(lb3d_batch *b, int cols, int lz, int threads)
 {
+    if (threads != 256 && threads != 512) return set_error(LB3D_EINVAL, "tile_threads must be 256 or 512");
     if (!b->B->cm_supported(b->C))
         return set_error(LB3D_EINVAL, "the collide-moments plan runs FP32 ternary batches");
-    if (cols < 1 || cols > 254 || lz < 1) return set_error(LB3D_EINVAL, "need 1 <= tile_cols <= 254 and lz >= 1");
+    if (cols < 1 || cols > threads - 2 || lz < 1)
+        return set_error(LB3D_EINVAL, "need 1 <= tile_cols <= tile_threads - 2 and lz >= 1");
     b->cm_cols = 0;
     b->cm_edge_sites = 0;
     std::vector<int> order(b->nlat);
@@ -192,10 +195,10 @@ int build_cm(lb3d_batch *b, int cols, int lz)
         int kx = (nx + cols / 2) / cols;
         if (kx < 1) kx = 1;
         int tx = (nx + kx - 1) / kx;
-        while (tx + 2 > 256) tx = (nx + ++kx - 1) / kx;
+        while (tx + 2 > threads) tx = (nx + ++kx - 1) / kx;
         kx = (nx + tx - 1) / tx;
         const int S = tx + 2;
-        int ty = 256 / S < ny ? 256 / S : ny;
+        int ty = threads / S < ny ? threads / S : ny;
         int ky = (ny + ty - 1) / ty;
         ty = (ny + ky - 1) / ky;
         ky = (ny + ty - 1) / ty;
@@ -249,6 +252,7 @@ int build_cm(lb3d_batch *b, int cols, int lz)
     b->edge_nitems = (uint32_t)eitems.size();
     b->cm_cols = cols;
     b->cm_lz = lz;
+    b->cm_threads = threads;
     b->aux_ok = false;
     return LB3D_OK;
 }
@@ -499,7 +503,8 @@ int lb3d_gpu_batch_set_plan(lb3d_batch *b, int32_t sequential, int32_t nwg, int3
     return drop_graph(b);
 }
 
-int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t lz, int32_t xcd_group)
+int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t tile_threads, int32_t lz,
+                                       int32_t xcd_group)
 {
     clear_error();
     if (!b) return set_error(LB3D_EINVAL, "NULL batch");
@@ -509,7 +514,7 @@ int lb3d_gpu_batch_set_collide_moments(lb3d_batch *b, int32_t tile_cols, int32_t
         b->cm_cols = 0;
         return LB3D_OK;
     }
-    if (int rc = build_cm(b, tile_cols, lz)) return rc;
+    if (int rc = build_cm(b, tile_cols, lz, tile_threads)) return rc;
     b->xg = (uint32_t)xcd_group;
     b->B->xg = b->xg;
     return LB3D_OK;
@@ -537,16 +542,19 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         unsigned nwg;
         uint32_t xg;
         uint64_t gb;     // group bytes of the grouped persistent plan (0: one group)
-        int cols = 0;    // collide-moments plan: tile columns (0: off) and z-chunk
+        int cols = 0;    // collide-moments plan: tile columns (0: off), z-chunk, work-group size
         int lz = 0;
+        int thr = 256;
     };
     std::vector<Cand> cands = {{true, 0, 0, 0}};
     for (unsigned g : {256u, 512u, 1024u, 2048u, 4096u, 8192u, 16384u})
         if (g < b->nitems) cands.push_back({false, g, 0, 0});
     cands.push_back({false, xcd_round(b->nitems), 0, 0});
     if (b->B->cm_supported(b->C))
-        for (int cols : {14, 16, 24, 30})
-            for (int lz : {16, 32, 64}) cands.push_back({false, 0, 16, 0, cols, lz});
+        for (int thr : {256, 512})
+            for (int cols : {30, 46, 62, 94})
+                for (int lz : {8, 16})
+                    if (cols <= thr - 2) cands.push_back({false, 0, 16, 0, cols, lz, thr});
     // the trials overwrite the dipole fields (the dipole pass writes them in
     // place); the guard restores them and the plan if a trial fails, and
     // frees the copy and the events on every path
@@ -556,7 +564,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         unsigned nwg;
         uint32_t xg;
         uint64_t gb;
-        int cols, lz;
+        int cols, lz, thr;
         size_t db_bytes = 0;
         void *db_save = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -570,7 +578,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
                 b->group_bytes = gb;
                 make_groups(b);
                 if (cols) {
-                    if (build_cm(b, cols, lz) == LB3D_OK) b->B->xg = xg;
+                    if (build_cm(b, cols, lz, thr) == LB3D_OK) b->B->xg = xg;
                 } else {
                     b->cm_cols = 0;
                 }
@@ -583,7 +591,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
             if (e1) (void)hipEventDestroy(e1);
             if (db_save) (void)hipFree(db_save);
         }
-    } g{b, b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz};
+    } g{b, b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz, b->cm_threads};
     // the trials overwrite aux(0), the scratch of the other plans
     b->aux_ok = false;
     g.db_bytes = b->C == 3 ? (b->db_off.back() + 3 * b->ns.back()) * b->rs : 0;
@@ -595,7 +603,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     BTRY(hipEventCreate(&g.e1));
     hipEvent_t e0 = g.e0, e1 = g.e1;
     double best = 1e300;
-    Cand bc = {b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz};
+    Cand bc = {b->sequential, b->nwg, b->xg, b->group_bytes, b->cm_cols, b->cm_lz, b->cm_threads};
     std::string rep;
     auto time_cand = [&](const Cand &c) -> int {
         b->sequential = c.seq;
@@ -604,8 +612,8 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         b->group_bytes = c.gb;
         make_groups(b);
         if (c.cols) {
-            if (c.cols != b->cm_cols || c.lz != b->cm_lz)
-                if (int rc = build_cm(b, c.cols, c.lz)) return rc;
+            if (c.cols != b->cm_cols || c.lz != b->cm_lz || c.thr != b->cm_threads)
+                if (int rc = build_cm(b, c.cols, c.lz, c.thr)) return rc;
             prime_aux(b);
         } else {
             b->cm_cols = 0;
@@ -625,7 +633,8 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         double per = ms / trial_steps;
         char buf[128];
         if (c.cols)
-            std::snprintf(buf, sizeof buf, "collide-moments cols=%d lz=%d xg=%u %.4f ms\n", c.cols, c.lz, c.xg, per);
+            std::snprintf(buf, sizeof buf, "collide-moments cols=%d threads=%d lz=%d xg=%u %.4f ms\n", c.cols, c.thr,
+                          c.lz, c.xg, per);
         else
             std::snprintf(buf, sizeof buf, "%s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
                           c.seq ? "sequential" : "persistent", c.nwg, c.xg, c.seq ? (size_t)0 : b->groups.size() - 1,
@@ -643,7 +652,7 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         const Cand top = bc;
         for (uint32_t xg : {1u, 4u, 16u, 64u, 256u})
             if (xg != top.xg)
-                if (int rc = time_cand({false, top.nwg, xg, 0, top.cols, top.lz})) return rc;
+                if (int rc = time_cand({false, top.nwg, xg, 0, top.cols, top.lz, top.thr})) return rc;
     }
     if (!bc.seq && !bc.cols) {  // then groups of lattices sized for the memory-side cache
         const Cand top = bc;
@@ -667,8 +676,8 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     b->group_bytes = bc.gb;
     make_groups(b);
     if (bc.cols) {
-        if (bc.cols != b->cm_cols || bc.lz != b->cm_lz)
-            if (int rc = build_cm(b, bc.cols, bc.lz)) return rc;
+        if (bc.cols != b->cm_cols || bc.lz != b->cm_lz || bc.thr != b->cm_threads)
+            if (int rc = build_cm(b, bc.cols, bc.lz, bc.thr)) return rc;
     } else {
         b->cm_cols = 0;
     }
@@ -676,8 +685,8 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
     drop_graph(b);
     char buf[128];
     if (bc.cols)
-        std::snprintf(buf, sizeof buf, "chosen: collide-moments cols=%d lz=%d xg=%u %.4f ms\n", bc.cols, bc.lz, bc.xg,
-                      best);
+        std::snprintf(buf, sizeof buf, "chosen: collide-moments cols=%d threads=%d lz=%d xg=%u %.4f ms\n", bc.cols,
+                      bc.thr, bc.lz, bc.xg, best);
     else
         std::snprintf(buf, sizeof buf, "chosen: %s nwg=%u xg=%u groups=%zu (%llu MiB) %.4f ms\n",
                       bc.seq ? "sequential" : "persistent", bc.nwg, bc.xg, bc.seq ? (size_t)0 : b->groups.size() - 1,

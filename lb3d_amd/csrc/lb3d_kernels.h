@@ -1722,16 +1722,17 @@ __device__ __forceinline__ void put_aux3(const Phys<R> &P, R *__restrict__ aux, 
     vec[s] = V4<R>{rho[1] - rho[0], da0, da1, da2};
 }
 
-template <typename R, bool Dev, int BD>
-__global__ __launch_bounds__(256) void k_batch_cm(const BatchLat<R> *__restrict__ lat,
-                                                  const uint4 *__restrict__ items, uint32_t nitems,
-                                                  const R *__restrict__ f, const R *__restrict__ aux,
-                                                  const R *__restrict__ db, R *__restrict__ fout,
-                                                  R *__restrict__ aux_out, uint32_t xg)
+template <typename R, bool Dev, int BD, int NTHR>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_cm(
+    const BatchLat<R> *__restrict__ lat, const uint4 *__restrict__ items, uint32_t nitems, const R *__restrict__ f,
+    const R *__restrict__ aux, const R *__restrict__ db, R *__restrict__ fout, R *__restrict__ aux_out, uint32_t xg)
 {
     static_assert(Dev, "the collide-moments plan runs the FP32 deviation form (ZORD order)");
-    __shared__ R xb[3][CM_NXY][256];  // post-collision populations of plane k
-    __shared__ R dbs[3][256];         // dipole field of plane k
+    // post-collision populations of plane k, two components at a time (the
+    // red and blue ones, then the surfactant's): 140 B per thread, so that
+    // four work-groups of 256 threads (two of 512) share a CU
+    __shared__ R xb[2][CM_NXY][NTHR];
+    __shared__ R dbs[3][NTHR];  // dipole field of plane k
     const int tid = threadIdx.x;
     const uint32_t n8 = xcd_round(nitems);
     for (uint32_t t = blockIdx.x; t < n8; t += gridDim.x) {
@@ -1761,10 +1762,34 @@ __global__ __launch_bounds__(256) void k_batch_cm(const BatchLat<R> *__restrict_
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int b = 0; b < 3; b++) ar[a][b] = ad[a][b] = R(0);
+        // one component's gather from plane k: LDS buffer u for the
+        // directions with an in-plane component, the site's own registers
+        // for the others
+        auto gather = [&](int c, int u, const R (&own_z)[3], const R (&dbo)[3]) __attribute__((always_inline)) {
+            int tq = tid, sq = S;  // opaque: the source slots are computed here, not hoisted
+            asm volatile("" : "+v"(tq), "+s"(sq));
+#pragma unroll
+            for (int ii = 0; ii < 19; ii++) {
+                const int i = ZORD[ii];
+                const int q = tq - CX[i] - CY[i] * sq;  // source region slot
+                const R v = CM_SLOT[i] < 0 ? own_z[i == 4 ? 0 : (i == 5 ? 1 : 2)] : xb[u][CM_SLOT[i] < 0 ? 0 : CM_SLOT[i]][q];
+                const int d = CZ[i] + 1;  // destination plane k - 1, k or k + 1
+                ar[c][d] = ar[c][d] + v * R(GD[i]);
+                if (c == 2) {
+                    const R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
+                    const R ng = ns_ * R(GD[i]);
+                    const R b0 = CM_SLOT[i] < 0 ? dbo[0] : dbs[0][q];
+                    const R b1 = CM_SLOT[i] < 0 ? dbo[1] : dbs[1][q];
+                    const R b2 = CM_SLOT[i] < 0 ? dbo[2] : dbs[2][q];
+                    da_term<Dev>(ng, b0, b1, b2, ad[d][0], ad[d][1], ad[d][2]);
+                }
+            }
+        };
         for (int k = z0; k < z1; k++) {
             // the site's own populations of the directions without an
-            // in-plane component (4, 5 and the rest direction 18)
-            R nz[3][3];
+            // in-plane component (4, 5 and the rest direction 18), and the
+            // surfactant's others until they go to LDS
+            R nz[3][3], n2[CM_NXY];
             R dbx[3] = {R(0), R(0), R(0)};
             __syncthreads();  // every gather of plane k-1 is done with xb and dbs
             if (act) {
@@ -1794,13 +1819,17 @@ __global__ __launch_bounds__(256) void k_batch_cm(const BatchLat<R> *__restrict_
 #pragma unroll
                     for (int c = 0; c < 3; c++) acc_moments<R>(i, n[c][i], rho[c], mom[c]);
                 }
+                // plain stores: non-temporal ones measured 6-26% slower here
+                // (tile row segments narrower than, or straddling, cache lines)
                 bgk_relax<R, Dev, 3, BD, false>(
                     P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, rho, mom, F, xo, yo, k,
                     s, fo, own, [&](int c, int i, R nv) __attribute__((always_inline)) {
-                        if (CM_SLOT[i] >= 0)
-                            xb[c][CM_SLOT[i]][tid] = nv;
-                        else
+                        if (CM_SLOT[i] < 0)
                             nz[c][i == 4 ? 0 : (i == 5 ? 1 : 2)] = nv;
+                        else if (c < 2)
+                            xb[c][CM_SLOT[i] < 0 ? 0 : CM_SLOT[i]][tid] = nv;
+                        else
+                            n2[CM_SLOT[i] < 0 ? 0 : CM_SLOT[i]] = nv;
                     });
                 dbx[0] = ld_plane(dl, s);
                 dbx[1] = ld_plane(plane_of(dl, 1u, P.ns), s);
@@ -1810,30 +1839,17 @@ __global__ __launch_bounds__(256) void k_batch_cm(const BatchLat<R> *__restrict_
             }
             __syncthreads();
             if (gat) {
-                // an opaque copy of the thread index: the 16 source slots
-                // are computed here, not hoisted out of the z loop
-                int tq = tid, sq = S;
-                asm volatile("" : "+v"(tq), "+s"(sq));
+                gather(0, 0, nz[0], dbx);
+                gather(1, 1, nz[1], dbx);
+            }
+            __syncthreads();  // the red populations' buffer is free
+            if (act) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-#pragma unroll
-                    for (int ii = 0; ii < 19; ii++) {
-                        const int i = ZORD[ii];
-                        const int q = tq - CX[i] - CY[i] * sq;  // source region slot
-                        const R v = CM_SLOT[i] < 0 ? nz[c][i == 4 ? 0 : (i == 5 ? 1 : 2)]
-                                                   : xb[c][CM_SLOT[i] < 0 ? 0 : CM_SLOT[i]][q];
-                        const int d = CZ[i] + 1;  // destination plane k - 1, k or k + 1
-                        ar[c][d] = ar[c][d] + v * R(GD[i]);
-                        if (c == 2) {
-                            const R ns_ = Dev ? (i < 18 ? P.wrho0_move[2] : P.wrho0_rest[2]) + v : v;
-                            const R ng = ns_ * R(GD[i]);
-                            const R b0 = CM_SLOT[i] < 0 ? dbx[0] : dbs[0][q];
-                            const R b1 = CM_SLOT[i] < 0 ? dbx[1] : dbs[1][q];
-                            const R b2 = CM_SLOT[i] < 0 ? dbx[2] : dbs[2][q];
-                            da_term<Dev>(ng, b0, b1, b2, ad[d][0], ad[d][1], ad[d][2]);
-                        }
-                    }
-                }
+                for (int j = 0; j < CM_NXY; j++) xb[0][j][tid] = n2[j];
+            }
+            __syncthreads();
+            if (gat) {
+                gather(2, 0, nz[2], dbx);
                 // plane k-1 has received all its directions
                 if (k - 1 >= z0 + 1 && k - 1 <= z1 - 2) {
                     const R rho[3] = {ar[0][0], ar[1][0], ar[2][0]};

@@ -2,6 +2,8 @@
 // translation unit per storage form (lb3d_fp64.hip, lb3d_fp32.hip).
 #pragma once
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstring>
 
@@ -506,20 +508,30 @@ class BatchLauncherT : public BatchLauncher {
         }
     }
     bool cm_supported(int ncomp) const override { return Dev && ncomp == 3; }
-    void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f, void *fout,
-                    const void *aux_in, void *aux_out, const void *db, hipStream_t s) override
+    void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, int threads, const void *f,
+                    void *fout, const void *aux_in, void *aux_out, const void *db, hipStream_t s) override
     {
         if constexpr (Dev) {
             const BatchLat<R> *L = (const BatchLat<R> *)lat;
             const uint4 *it = (const uint4 *)items;
             // one launch per equilibrium form present; a launch skips the
             // tiles of the other form
-            if (bd_mask & 1)
-                k_batch_cm<R, Dev, 0><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux_in,
-                                                            (const R *)db, (R *)fout, (R *)aux_out, xg);
-            if (bd_mask & 4)
-                k_batch_cm<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux_in,
-                                                            (const R *)db, (R *)fout, (R *)aux_out, xg);
+            auto go = [&](auto bd, auto nthr) {
+                constexpr int T = decltype(nthr)::value;
+                k_batch_cm<R, Dev, decltype(bd)::value, T><<<nwg, T, 0, s>>>(
+                    L, it, nitems, (const R *)f, (const R *)aux_in, (const R *)db, (R *)fout, (R *)aux_out, xg);
+            };
+            using T256 = std::integral_constant<int, 256>;
+            using T512 = std::integral_constant<int, 512>;
+            for (int bd : {0, 2}) {
+                if (!(bd_mask & (1 << bd))) continue;
+                if (bd == 0)
+                    threads == 512 ? go(std::integral_constant<int, 0>{}, T512{})
+                                   : go(std::integral_constant<int, 0>{}, T256{});
+                else
+                    threads == 512 ? go(std::integral_constant<int, 2>{}, T512{})
+                                   : go(std::integral_constant<int, 2>{}, T256{});
+            }
         }
     }
     void edge_moments(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *sites,

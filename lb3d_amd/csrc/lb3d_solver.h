@@ -85,11 +85,12 @@ class BatchLauncher {
     virtual void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg,
                        const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
     // the collide-moments plan (FP32 ternary only): k_batch_cm over nitems
-    // tiles (uint4 items), and k_batch_edge_moments over nitems runs of 256
+    // tiles (uint4 items) with work-groups of threads = 256 or 512, and k_batch_edge_moments over nitems runs of 256
     // edge sites (uint2 items into the site list)
     virtual bool cm_supported(int ncomp) const = 0;
-    virtual void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
-                            void *fout, const void *aux_in, void *aux_out, const void *db, hipStream_t s) = 0;
+    virtual void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, int threads,
+                            const void *f, void *fout, const void *aux_in, void *aux_out, const void *db,
+                            hipStream_t s) = 0;
     virtual void edge_moments(const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *sites,
                               const void *f, const void *db, void *aux, hipStream_t s) = 0;
 };

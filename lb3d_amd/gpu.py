@@ -139,7 +139,7 @@ def lib():
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
         L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip, ip]
-        L.lb3d_gpu_batch_set_collide_moments.argtypes = [vp, ip, ip, ip]
+        L.lb3d_gpu_batch_set_collide_moments.argtypes = [vp, ip, ip, ip, ip]
         L.lb3d_gpu_batch_set_groups.argtypes = [vp, ctypes.c_uint64]
         L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
@@ -462,12 +462,12 @@ class Batch:
         check(lib().lb3d_gpu_batch_set_plan(self._h, int(sequential), int(nwg), int(xcd_group)),
               "batch_set_plan")
 
-    def set_collide_moments(self, tile_cols=16, lz=32, xcd_group=16):
+    def set_collide_moments(self, tile_cols=62, lz=8, xcd_group=16, tile_threads=256):
         """The collide-moments plan (FP32 ternary): the collision gathers the
         next step's moments, populations are read once per step; tile_cols = 0
         turns it off."""
-        check(lib().lb3d_gpu_batch_set_collide_moments(self._h, int(tile_cols), int(lz), int(xcd_group)),
-              "batch_set_collide_moments")
+        check(lib().lb3d_gpu_batch_set_collide_moments(self._h, int(tile_cols), int(tile_threads), int(lz),
+                                                       int(xcd_group)), "batch_set_collide_moments")
 
     def set_groups(self, group_bytes):
         """Run the persistent plan in groups of lattices of about group_bytes

@@ -123,8 +123,9 @@ def cm_batch(forms, seed0=50):
     return b, per
 
 
-@pytest.mark.parametrize("cols,lz", [(16, 32), (14, 1), (30, 2), (6, 5), (254, 64)])
-def test_collide_moments_plan_equals_three_pass_bitwise(cols, lz):
+@pytest.mark.parametrize("cols,lz,thr", [(16, 32, 256), (14, 1, 256), (30, 2, 512), (6, 5, 256), (254, 64, 256),
+                                         (62, 8, 512), (510, 3, 512)])
+def test_collide_moments_plan_equals_three_pass_bitwise(cols, lz, thr):
     """The collide-moments plan (moments gathered from the post-collision
     populations, edge sites pulled) against the persistent three-pass plan
     and single-lattice runs: lattices that do not divide into tiles, z-chunks
@@ -135,7 +136,7 @@ def test_collide_moments_plan_equals_three_pass_bitwise(cols, lz):
     ref.set_plan(0, 64, 1)
     ref.time_loop(7)
     b, _ = cm_batch(forms)
-    b.set_collide_moments(cols, lz, 4)
+    b.set_collide_moments(cols, lz, 4, tile_threads=thr)
     b.time_loop(3)
     b.time_loop(4)
     assert b.nt == 7
@@ -163,7 +164,7 @@ def test_collide_moments_plan_switches_and_reinit():
     b.set_plan(0, 64, 1)
     b.time_loop(1)
     b.init_noise(2, 77, amp=(0.02, 0.02, 0.01))
-    b.set_collide_moments(24, 4, 1)
+    b.set_collide_moments(24, 4, 1, tile_threads=512)
     b.time_loop(3)
     b.autotune(1)
     b.set_collide_moments(16, 32, 16)
