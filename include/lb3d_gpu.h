@@ -234,6 +234,25 @@ int lb3d_gpu_profile_steps(lb3d_ctx *ctx, int32_t nsteps, int32_t max_kernels,
                            double *ms_per_step);
 
 /* ---------------------------------------------------------------------------
+ * One timestep that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  A
+ * periodic lattice of parameters p advances one step of the two-pass plan
+ * from f_in to f_out (the populations in the device layout of p->precision:
+ * f[(c*19 + i)*ns + s], FP64 values or FP32 deviations n_i - w_i*rho0_c;
+ * s = x + nx*(y + ny*z)); ternary lattices also read the dipole field db_in
+ * and write the relaxed field to db_out (db[k*ns + s]).  All four are
+ * caller-owned device buffers; the moments field lives in a static device
+ * array of LB3D_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends,
+ * and lattices whose moments need more return LB3D_EINVAL.  The step is
+ * enqueued on stream (a hipStream_t; NULL = the default stream) and returns
+ * without waiting.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams, other threads) race. */
+#define LB3D_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int lb3d_gpu_step_noalloc(const lb3d_params *p, const void *f_in, void *f_out, const void *db_in,
+                          void *db_out, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Batched ensemble (not a reference routine: the reference runs one lattice
  * per job).  Many independent periodic lattices of uneven sizes -- parameter
  * sweeps, seeds -- advanced together: each timestep is one persistent launch

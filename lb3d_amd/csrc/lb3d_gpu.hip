@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -1100,6 +1101,83 @@ int lb3d_gpu_le_state(lb3d_ctx *c, double *shear_sum, int32_t *lelo, double *lef
     *shear_u = c->host_le.shear_u;
     return LB3D_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// lb3d_gpu_step_noalloc: one timestep over caller-owned device buffers that
+// allocates nothing.  The moments field of the two-pass step (the scratch
+// of lb3d_gpu_run_steps' kernels) lives in a file-scope device array of
+// LB3D_NOALLOC_SCRATCH_BYTES, which every GPU the library runs on holds
+// until the process ends.  In host code `scratch` names a host-side stand-in;
+// its device address is looked up once per device and used instead.
+static __device__ __align__(256) unsigned char scratch[LB3D_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device.
+int scratch_on_device(void **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(LB3D_EINVAL, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(LB3D_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = ptr;
+    }
+    *out = g_scratch[dev];
+    return LB3D_OK;
+}
+}  // namespace
+
+extern "C" int lb3d_gpu_step_noalloc(const lb3d_params *p, const void *f_in, void *f_out, const void *db_in,
+                                     void *db_out, void *stream)
+{
+    g_err.clear();
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!f_in || !f_out || f_in == f_out) return fail(LB3D_EINVAL, "f_in and f_out must be distinct device buffers");
+    if (p->boundary != LB3D_BC_PERIODIC)
+        return fail(LB3D_EINVAL, "lb3d_gpu_step_noalloc runs periodic lattices (Lees-Edwards needs face planes)");
+    if (p->ncomp == 3 && (!db_in || !db_out || db_in == db_out))
+        return fail(LB3D_EINVAL, "ternary lattices need distinct db_in and db_out device buffers");
+    const uint64_t ns = (uint64_t)p->nx * p->ny * p->nz;
+    const size_t rs = p->precision == LB3D_FP64 ? 8 : 4;
+    const uint64_t need = aux_elems(p->ncomp, ns) * rs;
+    if (need > LB3D_NOALLOC_SCRATCH_BYTES) {
+        char m[200];
+        std::snprintf(m, sizeof m,
+                      "lattice needs %llu bytes of scratch, lb3d_gpu_step_noalloc has %llu: use lb3d_gpu_create "
+                      "and lb3d_gpu_run_steps",
+                      (unsigned long long)need, (unsigned long long)LB3D_NOALLOC_SCRATCH_BYTES);
+        return fail(LB3D_EINVAL, m);
+    }
+    void *aux = nullptr;
+    if ((rc = scratch_on_device(&aux))) return rc;
+    DevBufs b;
+    b.f[0] = const_cast<void *>(f_in);
+    b.f[1] = f_out;
+    b.cur = 0;
+    b.aux = aux;  // written in full by the moments pass before anything reads it
+    b.db = const_cast<void *>(db_in);
+    b.db_alt = db_out;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->precision == LB3D_FP64)
+        step_buffers_fp64(*p, b, s);
+    else
+        step_buffers_fp32(*p, b, s);
+    HIP_TRY(hipGetLastError());
+    return LB3D_OK;
+}
+
+extern "C" {
 
 int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, const char **names, double *ms,
                            int32_t *nkernels, double *ms_per_step)

@@ -96,6 +96,9 @@ class BatchLauncher {
 };
 
 Launcher *make_launcher_fp64(const lb3d_params &p);
+// one two-pass timestep over caller-provided buffers (no allocation)
+void step_buffers_fp64(const lb3d_params &p, const DevBufs &b, hipStream_t s);
+void step_buffers_fp32(const lb3d_params &p, const DevBufs &b, hipStream_t s);
 Launcher *make_launcher_fp32(const lb3d_params &p);
 BatchLauncher *make_batch_launcher_fp64();
 BatchLauncher *make_batch_launcher_fp32();

@@ -41,7 +41,7 @@ EXPORTS = [
     "lb3d_gpu_batch_upload_sites", "lb3d_gpu_batch_run_steps", "lb3d_gpu_batch_get_nt",
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
     "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_set_groups", "lb3d_gpu_batch_autotune",
-    "lb3d_gpu_batch_profile_steps", "lb3d_gpu_batch_set_collide_moments",
+    "lb3d_gpu_batch_profile_steps", "lb3d_gpu_batch_set_collide_moments", "lb3d_gpu_step_noalloc",
 ]
 
 
@@ -140,6 +140,7 @@ def lib():
         L.lb3d_gpu_batch_diagnostics.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.lb3d_gpu_batch_set_plan.argtypes = [vp, ip, ip, ip]
         L.lb3d_gpu_batch_set_collide_moments.argtypes = [vp, ip, ip, ip, ip]
+        L.lb3d_gpu_step_noalloc.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp]
         L.lb3d_gpu_batch_set_groups.argtypes = [vp, ctypes.c_uint64]
         L.lb3d_gpu_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
         L.lb3d_gpu_batch_profile_steps.argtypes = [vp, ip, dp, dp]
