@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
 """bench.py -- MLUPS and HBM-roofline fraction of the LB3D D3Q19 timestep on
-one MI355X, FP32 deviation storage against the plain FP64 path.
+one MI355X: the FP32-batched ensemble against the plain FP64 path.
 
-Workload (the spinodal-decomposition configuration): binary immiscible
-Shan-Chen, D3Q19 BGK, 256^3 periodic, tau = 1, g_br = 0.08, bdist 2, initial
-densities 0.7 + U(-0.01, 0.01) per component and site (seed 1), zero velocity.
-Both forms start from the same device-generated state; the timed region is
-`--steps` whole timesteps (graph replay) with the lattice resident in HBM,
-bracketed by stream synchronisation.  `value` is the FP32-deviation MLUPS;
-the FP64-plain MLUPS, the speedup and the density error between the two are
-reported beside it.  Per-kernel durations for the roofline come from HIP
-events on the library's own stream.  The CPU baseline is the oracle (the FP64
-restatement of the reference loop, one core) timed on a bounded sample.
+Headline workload (the batched ternary ensemble): 48 independent periodic
+ternary lattices, cube edges drawn uniformly from {16, 24, ..., 96} (numpy
+default_rng(1)), g_br uniform in [0.06, 0.12], g_bs = -0.006, g_ss = -0.003,
+tau = tau_d = 1, beta = 10, d_0 = 1, bdist 2, densities 0.7 + U(-0.01, 0.01)
+for red and blue, 0.7 surfactant, dipoles uniform on the sphere (noise seed
+1000 + k), FP32 deviation storage, all lattices advanced together by the
+tuner's plan with graph-replayed steps.  The timed region is `--steps` whole
+steps with every lattice resident in HBM, bracketed by stream
+synchronisation; `value` is its MLUPS (sum of lattice sites x steps / wall
+time).  Beside it: the 48 lattices run one after another on the plain FP64
+path, the step's rate against the 480 B/site roofline with HIP-event timing
+on the batch's stream, the HBM traffic per step from rocprofv3 counters, the
+per-phase kernel times, and the FP32 error against FP64.
+
+The binary spinodal lattice (256^3, FP32 deviation and FP64 plain), the
+ternary 128^3 and the Lees-Edwards 256x64x64 configurations are reported in
+extra keys, each with its FP32-against-FP64 error, the FP64 floor (a 2^-51
+perturbation) and an FP64 control perturbed at FP32 resolution (2^-24).  The
+CPU baseline is the oracle (the FP64 restatement of the reference loop)
+timed on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -57,12 +67,13 @@ def errs(a, b):
     return float(d.max() / ref.max()), float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))
 
 
-def run_form(n, ncomp, precision, steps, warmup, profile_steps, rho_shift=0.0):
+def run_form(n, ncomp, precision, steps, warmup, profile_steps):
+    """The binary spinodal lattice in one storage form: the tuner's plan,
+    `steps` timed steps (wall clock and HIP events), per-kernel times."""
     from lb3d_amd.gpu import Lattice
 
     g = Lattice(n, n, n, ncomp=ncomp, precision=precision, g_br=0.08, bdist=2)
-    m = 0.7 + rho_shift
-    g.init_noise(1, rho_mean=(m, m, m), amp=(0.01, 0.01, 0.0), dipoles=False)
+    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
     g.bc_init()
     report = g.autotune(trial_steps=3)
     chosen = report.strip().splitlines()[-1]
@@ -72,64 +83,105 @@ def run_form(n, ncomp, precision, steps, warmup, profile_steps, rho_shift=0.0):
     t0 = time.perf_counter()
     ev_ms = g.time_loop_timed(steps)
     dt = time.perf_counter() - t0
-    rho = g.density()
-    u = velocity(rho, g.momentum())
     kern, ms_step_prof = g.profile_steps(profile_steps)
-    m, _, counts = g.sanity_check()
     g.close()
     return dict(seconds=dt, ms_per_step=1e3 * dt / steps, mlups=n ** 3 * steps / dt / 1e6,
-                event_ms_per_step=ev_ms / steps, kernels=kern, plan=chosen, prof_ms_per_step=ms_step_prof,
-                rho=rho, u=u, mass=m.tolist(), counts=counts.tolist())
+                event_ms_per_step=ev_ms / steps, kernels=kern, plan=chosen, prof_ms_per_step=ms_step_prof)
 
 
-def config_accuracy(name, dims, steps, kw, amp):
-    """FP32 deviation form against FP64 on one configuration after `steps`
-    steps: density and velocity errors, the FP64 floor (FP64 against FP64 from
-    an initial mean density 2^-51 higher), and for Lees-Edwards the x-halo
+# The FP64 comparison runs: the floor starts from a mean density 2^-51
+# higher (uniform); the control from the noise of the initial state scaled so
+# that every site's density moves by about 0.7 x 2^-24 (random per site, like
+# FP32's rounding, which a uniform shift of that size does not mimic: it
+# barely excites the unstable modes).
+CONTROL_DRHO = 0.7 * 2.0 ** -24
+PERTURB = (("floor", 2.0 ** -51, "initial mean density x (1 + 2^-51)"),
+           ("control", CONTROL_DRHO, "initial noise amplitude a -> a + 0.7 x 2^-24 (a site's density moves by "
+                                     "up to 0.7 x 2^-24, FP32 resolution)"))
+
+
+def perturbed(tag, amp):
+    """(mean density, noise amplitudes) of the run `tag`."""
+    if tag == "floor":
+        return 0.7 * (1.0 + PERTURB[0][1]), amp
+    if tag == "control":
+        return 0.7, tuple(a + CONTROL_DRHO if a > 0 else a for a in amp)
+    return 0.7, amp
+
+
+def first_over(traj, key, tol=1e-5):
+    """First checkpoint step at which the density error `key` exceeds tol (None: never)."""
+    return next((t["step"] for t in traj if t[key] > tol), None)
+
+
+def config_accuracy(name, dims, steps, kw, amp, checkpoints):
+    """FP32 deviation form against FP64 on one configuration, advanced
+    together to each checkpoint (the last one is `steps`): the density error
+    along the way, the FP64 floor (FP64 against FP64 from an initial mean
+    density 2^-51 higher) and an FP64 control perturbed at FP32 resolution
+    (2^-24), the velocity error at the end, and for Lees-Edwards the x-halo
     planes and the mean shear stress."""
     from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64, Lattice
 
     le = kw.get("boundary") == LB3D_BC_LEES_EDWARDS
-    res = {}
-    for tag, prec, shift in (("fp64", LB3D_FP64, 0.0), ("fp32", LB3D_FP32_DEV, 0.0),
-                             ("floor", LB3D_FP64, 2.0 ** -51)):
+    runs, secs = {}, {}
+    for tag, prec in (("fp64", LB3D_FP64), ("fp32", LB3D_FP32_DEV), ("floor", LB3D_FP64), ("control", LB3D_FP64)):
         g = Lattice(*dims, precision=prec, **kw)
-        m = 0.7 + shift
-        g.init_noise(1, rho_mean=(m, m, m), amp=amp, dipoles=True)
+        m, a = perturbed(tag, amp)
+        g.init_noise(1, rho_mean=(m, m, m), amp=a, dipoles=True)
         g.bc_init()
         g.autotune(trial_steps=2)
-        t0 = time.perf_counter()
-        g.time_loop(steps)
-        g.synchronize()
-        dt = time.perf_counter() - t0
+        runs[tag], secs[tag] = g, 0.0
+    traj, done = [], 0
+    for cp in list(checkpoints) + [steps]:
+        if cp <= done or cp > steps:
+            continue
+        for tag, g in runs.items():
+            t0 = time.perf_counter()
+            g.time_loop(cp - done)
+            g.synchronize()
+            secs[tag] += time.perf_counter() - t0
+        done = cp
+        rho = {tag: g.density() for tag, g in runs.items()}
+        traj.append(dict(step=cp, **{tag: errs(rho[tag], rho["fp64"])[0] for tag in ("fp32", "floor", "control")}))
+    res = {}
+    for tag, g in runs.items():
         rho = g.density()
-        r = dict(rho=rho, u=velocity(rho, g.momentum()), mlups=float(np.prod(dims)) * steps / dt / 1e6,
-                 mass=g.sanity_check()[0][:kw["ncomp"]])
+        r = dict(rho=rho, u=velocity(rho, g.momentum()), mass=g.sanity_check()[0][:kw["ncomp"]])
         if le:
             r["halo"] = g.le_halo()
             r["stress"] = g.shear_stress()[2]
         res[tag] = r
         g.close()
-    a, b, f = res["fp32"], res["fp64"], res["floor"]
+    a, b = res["fp32"], res["fp64"]
     out = dict(steps=steps, lattice=list(dims),
-               mlups=dict(fp64=round(b["mlups"], 1), fp32=round(a["mlups"], 1)))
+               mlups={t: round(float(np.prod(dims)) * steps / secs[t] / 1e6, 1) for t in ("fp64", "fp32")})
     out["density_linf_rel"], out["density_l2_rel"] = errs(a["rho"], b["rho"])
     out["velocity_linf_rel"], out["velocity_l2_rel"] = errs(a["u"], b["u"])
-    fl = errs(f["rho"], b["rho"])
-    out["fp64_perturbed_floor"] = dict(density_linf_rel=fl[0], density_l2_rel=fl[1],
-                                       velocity_linf_rel=errs(f["u"], b["u"])[0])
+    for tag, rel, what in PERTURB:
+        f = res[tag]
+        fl = errs(f["rho"], b["rho"])
+        key = "fp64_perturbed_floor" if tag == "floor" else "fp64_control_fp32_resolution"
+        out[key] = dict(density_linf_rel=fl[0], density_l2_rel=fl[1],
+                        velocity_linf_rel=errs(f["u"], b["u"])[0], perturbation=what)
     # pointwise errors of an unstable flow grow from the rounding of each
-    # step; the ratio to the floor run (a 2^-51 perturbation) says how much of
-    # that growth is the flow's own, and the spread of the density field is a
-    # statistic that survives it
-    out["density_linf_over_floor"] = out["density_linf_rel"] / max(fl[0], 1e-300)
+    # step; the control run says how far a perturbation of FP32's size grows
+    # on its own, the floor run how far one of FP64's size does
+    out["density_linf_over_floor"] = out["density_linf_rel"] / max(out["fp64_perturbed_floor"]["density_linf_rel"],
+                                                                   1e-300)
+    out["density_linf_over_control"] = out["density_linf_rel"] / max(
+        out["fp64_control_fp32_resolution"]["density_linf_rel"], 1e-300)
+    out["trajectory_density_linf"] = traj
+    out["first_step_density_linf_over_1e-5"] = {t: first_over(traj, t) for t in ("fp32", "control", "floor")}
     out["density_std_rel"] = float(abs(a["rho"][0].std() - b["rho"][0].std()) / b["rho"][0].std())
     out["mass_rel"] = float(np.max(np.abs(a["mass"] - b["mass"]) / np.abs(b["mass"])))
     if le:
         out["le_halo_linf_rel"] = errs(a["halo"], b["halo"])[0]
         out["mean_shear_stress"] = dict(fp64=b["stress"], fp32=a["stress"],
                                         rel_diff=abs(a["stress"] - b["stress"]) / abs(b["stress"]))
-    print(f"[bench] {name}: density Linf {out['density_linf_rel']:.3e} (floor {fl[0]:.3e})",
+    print(f"[bench] {name}: density Linf {out['density_linf_rel']:.3e} (control "
+          f"{out['fp64_control_fp32_resolution']['density_linf_rel']:.3e}, floor "
+          f"{out['fp64_perturbed_floor']['density_linf_rel']:.3e}) after {steps} steps; trajectory {traj}",
           file=sys.stderr, flush=True)
     return out
 
@@ -174,22 +226,174 @@ def ternary_probe(n):
     g.close()
 
 
-def ensemble_probe(groups_mib):
-    """Child run for rocprofv3 passes: the 48-lattice FP32 ternary ensemble on
-    the persistent plan (one work item per workgroup, XCD chunks of 16 items)
-    in groups of groups_mib MiB of populations (0: one group), 3 steps."""
-    from lb3d_amd.gpu import LB3D_FP32_DEV, Batch
+def le_probe(precision_name):
+    """Child run for rocprofv3 passes: the Lees-Edwards 256x64x64 binary lattice
+    (shear velocity 0.05, g_br 0.08) in the given form on the tuner's plan,
+    5 steps, then a device copy of known size (the counter calibration)."""
+    import torch  # noqa: F401  (first: see copy_calibration)
+    from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64, Lattice
 
-    edges, g_br = ensemble_spec(48)
-    b = Batch([(e, e, e) for e in edges], per_lattice=[dict(g_br=g) for g in g_br], precision=LB3D_FP32_DEV,
-              ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
-    for k in range(48):
-        b.init_noise(k, 1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
-    b.set_plan(0, nwg=1 << 30, xcd_group=16)
-    b.set_groups(groups_mib << 20)
-    b.time_loop(3)
+    prec = LB3D_FP32_DEV if precision_name == "fp32" else LB3D_FP64
+    g = Lattice(256, 64, 64, ncomp=2, precision=prec, g_br=0.08, bdist=2, boundary=LB3D_BC_LEES_EDWARDS,
+                shear_u=0.05)
+    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
+    g.bc_init()
+    print(g.autotune(trial_steps=3).strip().splitlines()[-1], flush=True)
+    g.time_loop(5)
+    g.synchronize()
+    print(g.profile_steps(5), flush=True)
+    g.close()
+    copy_calibration()
+
+
+ENS_COMMON = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
+ENS_BYTES = 480  # bytes per site update: 3 x 19 FP32 populations and the 3-vector dipole, read and written
+
+
+def ensemble_batch(precision, nlat=48, tag="fp64"):
+    """The ensemble's lattices with their seeded initial state (perturbed as
+    the comparison run `tag` is)."""
+    from lb3d_amd.gpu import Batch
+
+    edges, g_br = ensemble_spec(nlat)
+    b = Batch([(e, e, e) for e in edges], per_lattice=[dict(g_br=g) for g in g_br], precision=precision,
+              **ENS_COMMON)
+    m, a = perturbed(tag, (0.01, 0.01, 0.0))
+    for k in range(nlat):
+        b.init_noise(k, 1000 + k, rho_mean=(m, m, m), amp=a, dipoles=True)
+    return b, edges
+
+
+def batch_plan(chosen):
+    """The batch tuner's 'chosen: ...' line as arguments of apply_batch_plan."""
+    w = chosen.split()
+    kv = {a.split("=")[0]: int(a.split("=")[1]) for a in w if "=" in a}
+    if w[1] == "persistent":
+        return dict(kind="persistent", nwg=kv["nwg"], xg=kv["xg"], group_mib=int(chosen.split("(")[1].split()[0]))
+    if w[1] == "collide-moments":
+        return dict(kind="collide-moments", cols=kv["cols"], threads=kv["threads"], lz=kv["lz"], xg=kv["xg"])
+    return dict(kind="sequential")
+
+
+def apply_batch_plan(b, plan):
+    if plan["kind"] == "persistent":
+        b.set_plan(0, plan["nwg"], plan["xg"])
+        b.set_groups(plan["group_mib"] << 20)
+    elif plan["kind"] == "collide-moments":
+        b.set_collide_moments(plan["cols"], plan["lz"], plan["xg"], tile_threads=plan["threads"])
+    else:
+        b.set_plan(1)
+
+
+PROBE_STEPS = 4
+
+
+def ensemble_probe(plan):
+    """Child run under rocprofv3: PROBE_STEPS steps of the FP32 ensemble on
+    the given plan, then a device copy of known size (the counter
+    calibration)."""
+    import torch  # noqa: F401  (first: see copy_calibration)
+    from lb3d_amd.gpu import LB3D_FP32_DEV
+
+    b, _ = ensemble_batch(LB3D_FP32_DEV)
+    apply_batch_plan(b, plan)
+    b.time_loop(PROBE_STEPS)
     b.synchronize()
     b.close()
+    copy_calibration()
+
+
+def copy_calibration():
+    # torch must be imported before lb3d_amd initialises the HIP runtime (the
+    # probes import it first); a late first import finds no GPU
+    import torch
+
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def counter_sums(outdir, counter):
+    """Sum and count of `counter` over the dispatches of each kernel, from a rocprofv3 database."""
+    import glob
+    import sqlite3
+
+    dbs = glob.glob(os.path.join(outdir, "**", "*.db"), recursive=True)
+    if not dbs:
+        return {}
+    con = sqlite3.connect(dbs[0])
+    acc = {}
+    for name, val in con.execute("select kernel_name, value from counters_collection where counter_name=?",
+                                 (counter,)):
+        t = acc.setdefault(name, [0.0, 0])
+        t[0] += val
+        t[1] += 1
+    con.close()
+    return acc
+
+
+def run_pmc(counter, probe_args, timeout=180):
+    """One rocprofv3 --pmc pass of `counter` over `bench.py <probe_args>`."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    out = tempfile.mkdtemp(prefix="lb3d_pmc_")
+    cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "pmc", "--", sys.executable,
+           os.path.abspath(__file__)] + probe_args
+    print(f"[bench] traffic: {' '.join(cmd)}", file=sys.stderr, flush=True)
+    p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True)
+    try:
+        rc = p.wait(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        shutil.rmtree(out, ignore_errors=True)
+        return None, f"rocprofv3 --pmc {counter} timed out"
+    sums = counter_sums(out, counter)
+    shutil.rmtree(out, ignore_errors=True)
+    if not sums:
+        return None, f"rocprofv3 --pmc {counter}: rc {rc}, no counters"
+    return sums, None
+
+
+def measure_ensemble_traffic(plan):
+    """HBM bytes per step of the ensemble on `plan`, per kernel and in total:
+    the kernels of PROBE_STEPS steps under rocprofv3 --pmc FETCH_SIZE, then
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a
+    device copy of COPY_BYTES each way."""
+    import shutil
+
+    if not shutil.which("rocprofv3"):
+        return None, "rocprofv3 not found"
+    # the kernels of a step (the collide-moments plan's moments pass runs
+    # only to prime the carried moments)
+    step_kernels = (("k_batch_dipole", "k_batch_cm", "k_batch_edge_moments") if plan["kind"] == "collide-moments"
+                    else ("k_batch_moments", "k_batch_dipole", "k_batch_collide", "k_moments", "k_dipole",
+                          "k_collide"))
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        sums, why = run_pmc(counter, ["--ensemble-probe", json.dumps(plan)])
+        if sums is None:
+            return None, why
+        copy = [v[0] / v[1] for k, v in sums.items() if "lb3d" not in k and "copy" in k.lower()]
+        if not copy:
+            return None, f"rocprofv3 --pmc {counter}: no copy kernel among {sorted(sums)[:6]}"
+        scale = COPY_BYTES / (copy[0] * 1024.0)
+        per = {}
+        for k, (tot, n) in sums.items():
+            short = k.split("(")[0].split("<")[0].replace("void lb3d::", "").strip()
+            if short in step_kernels:
+                per[short] = per.get(short, 0.0) + tot * 1024.0 * scale / PROBE_STEPS
+        got[counter] = (per, scale)
+    fetch, fs = got["FETCH_SIZE"]
+    write, ws = got["WRITE_SIZE"]
+    kernels = {k: dict(read=round(fetch.get(k, 0.0)), write=round(write.get(k, 0.0))) for k in set(fetch) | set(write)}
+    total = sum(v["read"] + v["write"] for v in kernels.values())
+    return dict(bytes_per_step=total, kernels=kernels, fetch_scale=round(fs, 4), write_scale=round(ws, 4)), None
 
 
 def counter_means(outdir, counter):
@@ -276,87 +480,98 @@ def ensemble_spec(nlat=48, seed=1):
 
 
 def run_ensemble(steps, warmup, seq_steps, nlat=48):
-    """FP32 batched persistent launches against the lattices run one after
-    another on the plain FP64 path."""
-    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64, Batch
+    """The FP32 batched ensemble on the tuner's plan (`steps` timed steps after
+    `warmup`), against the lattices run one after another on the plain FP64
+    path."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64
 
-    edges, g_br = ensemble_spec(nlat)
-    dims = [(e, e, e) for e in edges]
-    per = [dict(g_br=g) for g in g_br]
-    common = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
+    b, edges = ensemble_batch(LB3D_FP32_DEV, nlat)
     sites = sum(e ** 3 for e in edges)
-    out = {}
-    for name, prec, nsteps in (("fp32_batched", LB3D_FP32_DEV, steps), ("fp64_sequential", LB3D_FP64, seq_steps)):
-        b = Batch(dims, per_lattice=per, precision=prec, **common)
-        for k in range(nlat):
-            b.init_noise(k, 1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
-        if prec == LB3D_FP64:
-            b.set_plan(1)   # lattice after lattice, the single-lattice FP64 kernels
-            chosen = "sequential"
-        else:
-            chosen = b.autotune(3).strip().splitlines()[-1]
-        b.time_loop(warmup)
-        b.synchronize()
-        t0 = time.perf_counter()
-        b.time_loop(nsteps)
-        b.synchronize()
-        dt = time.perf_counter() - t0
-        ph, mps = b.profile_steps(3)
-        out[name] = dict(mlups=sites * nsteps / dt / 1e6, ms_per_step=1e3 * dt / nsteps, plan=chosen,
-                         phases_ms=ph, steps=nsteps)
-        print(f"[bench] ensemble {name}: {out[name]['mlups']:.1f} MLUPS {chosen} {ph}", file=sys.stderr, flush=True)
-        b.close()
-    B = bytes_per_site(3, True, 4)
-    r = out["fp32_batched"]
-    return dict(lattices=nlat, sites=sites, edges=edges, value=round(r["mlups"], 1), unit="MLUPS",
-                ms_per_step=round(r["ms_per_step"], 4), steps=r["steps"], plan=r["plan"],
-                step_frac=round(r["mlups"] * 1e6 * B / (HBM_PEAK_GBS * 1e9), 4), bytes_per_site=B,
-                phases_ms={k: round(v, 4) for k, v in r["phases_ms"].items()},
-                fp64_sequential=dict(value=round(out["fp64_sequential"]["mlups"], 1), unit="MLUPS",
-                                     steps=out["fp64_sequential"]["steps"]),
-                speedup_vs_fp64_sequential=round(r["mlups"] / out["fp64_sequential"]["mlups"], 3))
+    chosen = b.autotune(3).strip().splitlines()[-1]
+    print(f"[bench] ensemble autotune: {chosen}", file=sys.stderr, flush=True)
+    b.time_loop(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = b.time_loop_timed(steps)
+    dt = time.perf_counter() - t0
+    ph, mps = b.profile_steps(3)
+    b.close()
+    fp32 = dict(mlups=sites * steps / dt / 1e6, ms_per_step=1e3 * dt / steps, event_ms_per_step=ev_ms / steps,
+                phases_ms={k: round(v, 4) for k, v in ph.items()}, profiled_ms_per_step=round(mps, 4), plan=chosen)
+    print(f"[bench] ensemble fp32_batched: {fp32['mlups']:.1f} MLUPS, {fp32['event_ms_per_step']:.4f} ms per step "
+          f"(HIP events) {ph}", file=sys.stderr, flush=True)
+    q, _ = ensemble_batch(LB3D_FP64, nlat)
+    q.set_plan(1)  # lattice after lattice, the single-lattice FP64 kernels
+    q.time_loop(2)
+    q.synchronize()
+    t0 = time.perf_counter()
+    q.time_loop(seq_steps)
+    q.synchronize()
+    dq = time.perf_counter() - t0
+    q.close()
+    fp64 = dict(value=round(sites * seq_steps / dq / 1e6, 1), unit="MLUPS", steps=seq_steps,
+                ms_per_step=round(1e3 * dq / seq_steps, 4), plan="sequential (one lattice after another)")
+    print(f"[bench] ensemble fp64_sequential: {fp64['value']} MLUPS", file=sys.stderr, flush=True)
+    return dict(lattices=nlat, sites=sites, edges=edges, fp32=fp32, fp64_sequential=fp64)
 
 
-def ensemble_accuracy(steps, nlat=48):
-    """The FP32 batched ensemble against the FP64 batched ensemble after `steps`
-    steps from the same initial state: density errors of every lattice (the
-    worst lattice reported), beside the FP64 floor (FP64 against FP64 started
-    from a mean density 2^-51 higher)."""
-    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64, Batch
+def ensemble_accuracy(steps, checkpoints, nlat=48):
+    """The FP32 batched ensemble against the FP64 batched ensemble from the
+    same initial state, advanced together to each checkpoint: density errors
+    of every lattice (the worst lattice reported) beside the FP64 floor
+    (2^-51) and an FP64 control perturbed at FP32 resolution (2^-24)."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64
 
     G = np.array([2.0] * 6 + [1.0] * 13)
-    edges, g_br = ensemble_spec(nlat)
-    dims = [(e, e, e) for e in edges]
-    per = [dict(g_br=g) for g in g_br]
-    common = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
-    rho = {}
-    for tag, prec, shift in (("fp64", LB3D_FP64, 0.0), ("fp32", LB3D_FP32_DEV, 0.0), ("floor", LB3D_FP64, 2.0 ** -51)):
-        b = Batch(dims, per_lattice=per, precision=prec, **common)
-        m = 0.7 + shift
-        for k in range(nlat):
-            b.init_noise(k, 1000 + k, rho_mean=(m, m, m), amp=(0.01, 0.01, 0.0), dipoles=True)
-        b.time_loop(steps)
-        b.synchronize()
-        # densities from the populations (the collision conserves them per site)
-        rho[tag] = [np.einsum("i,cizyx->czyx", G, b.populations(k)) for k in range(nlat)]
+    runs = {}
+    for tag, prec in (("fp64", LB3D_FP64), ("fp32", LB3D_FP32_DEV), ("floor", LB3D_FP64), ("control", LB3D_FP64)):
+        runs[tag], edges = ensemble_batch(prec, nlat, tag)
+
+    def densities(b):
+        # from the populations (the collision conserves them per site)
+        return [np.einsum("i,cizyx->czyx", G, b.populations(k)) for k in range(nlat)]
+
+    def worst(rho, ref):
+        linf = [float(np.abs(a - r).max() / np.abs(r).max()) for a, r in zip(rho, ref)]
+        return max(linf), linf
+
+    traj, done, final = [], 0, {}
+    for cp in list(checkpoints) + [steps]:
+        if cp <= done or cp > steps:
+            continue
+        for b in runs.values():
+            b.time_loop(cp - done)
+            b.synchronize()
+        done = cp
+        rho = {tag: densities(b) for tag, b in runs.items()}
+        traj.append(dict(step=cp, **{t: worst(rho[t], rho["fp64"])[0] for t in ("fp32", "floor", "control")}))
+        final = rho
+    for b in runs.values():
         b.close()
     out = {}
-    for tag in ("fp32", "floor"):
-        linf = [float(np.abs(a - r).max() / np.abs(r).max()) for a, r in zip(rho[tag], rho["fp64"])]
-        l2 = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(rho[tag], rho["fp64"])]
+    for tag in ("fp32", "floor", "control"):
+        a_, r_ = final[tag], final["fp64"]
+        _, linf = worst(a_, r_)
+        l2 = [float(np.linalg.norm(a - r) / np.linalg.norm(r)) for a, r in zip(a_, r_)]
         # statistics that survive the flow's amplification of rounding: the
         # spread of each density field and each component's mass
-        spread = [float(abs(a[0].std() - r[0].std()) / r[0].std()) for a, r in zip(rho[tag], rho["fp64"])]
+        spread = [float(abs(a[0].std() - r[0].std()) / r[0].std()) for a, r in zip(a_, r_)]
         mass = [float(np.max(np.abs(a.sum(axis=(1, 2, 3)) - r.sum(axis=(1, 2, 3))) / r.sum(axis=(1, 2, 3))))
-                for a, r in zip(rho[tag], rho["fp64"])]
+                for a, r in zip(a_, r_)]
         out[tag] = dict(density_linf_rel=max(linf), density_l2_rel=max(l2),
                         worst_lattice_edge=edges[int(np.argmax(linf))],
                         lattices_within_1e5_linf=int(sum(v <= 1e-5 for v in linf)),
                         density_std_rel=max(spread), mass_rel=max(mass))
-    res = dict(steps=steps, **out["fp32"], fp64_perturbed_floor=out["floor"])
+    res = dict(steps=steps, **out["fp32"])
+    res["fp64_perturbed_floor"] = dict(out["floor"], perturbation=PERTURB[0][2])
+    res["fp64_control_fp32_resolution"] = dict(out["control"], perturbation=PERTURB[1][2])
     res["density_linf_over_floor"] = res["density_linf_rel"] / max(out["floor"]["density_linf_rel"], 1e-300)
+    res["density_linf_over_control"] = res["density_linf_rel"] / max(out["control"]["density_linf_rel"], 1e-300)
+    res["trajectory_density_linf"] = traj
+    res["first_step_density_linf_over_1e-5"] = {t: first_over(traj, t) for t in ("fp32", "control", "floor")}
     print(f"[bench] ensemble accuracy after {steps} steps: density Linf {res['density_linf_rel']:.3e} "
-          f"L2 {res['density_l2_rel']:.3e} (floor {out['floor']['density_linf_rel']:.3e})", file=sys.stderr, flush=True)
+          f"(control {out['control']['density_linf_rel']:.3e}, floor {out['floor']['density_linf_rel']:.3e}); "
+          f"trajectory {traj}", file=sys.stderr, flush=True)
     return res
 
 
@@ -369,61 +584,79 @@ def host_threads():
     return len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(sample_n, sample_steps, single_n, single_steps):
+def cpu_baseline(sample_lattices, sample_steps, single_n, single_steps):
     """The oracle (FP64 C restatement of the reference loop) on a bounded
-    sample of the workload: all host threads, then one core."""
+    sample of the ensemble: its first `sample_lattices` lattices with edges up
+    to 56, `sample_steps` steps each, on all host threads; then one lattice of
+    edge single_n on one core."""
     from oracle.pyoracle import Oracle, set_threads
 
-    def timed(nn, steps, threads):
+    edges, g_br = ensemble_spec(48)
+    kw = dict(ENS_COMMON)
+
+    def timed(picks, steps, threads):
         set_threads(threads)
-        o = Oracle(nn, nn, nn, ncomp=2, g_br=0.08, bdist=2)
-        o.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
-        o.bc_init()
-        t0 = time.perf_counter()
-        o.run(steps)
-        dt = time.perf_counter() - t0
-        o.close()
-        return nn ** 3 * steps / dt / 1e6, dt
+        n_upd, t = 0, 0.0
+        for k, e in picks:
+            o = Oracle(e, e, e, g_br=g_br[k], **kw)
+            o.init_noise(1000 + k, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=True)
+            o.bc_init()
+            t0 = time.perf_counter()
+            o.run(steps)
+            t += time.perf_counter() - t0
+            n_upd += e ** 3 * steps
+            o.close()
+        return n_upd / t / 1e6, t
 
     th = host_threads()
-    v, dt = timed(sample_n, sample_steps, th)
-    v1, dt1 = timed(single_n, single_steps, 1)
+    picks = [(k, e) for k, e in enumerate(edges) if e <= 56][:sample_lattices]
+    v, dt = timed(picks, sample_steps, th)
+    k1 = next(k for k, e in enumerate(edges) if e == single_n) if single_n in edges else 0
+    v1, dt1 = timed([(k1, single_n)], single_steps, 1)
     cpu = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
     model = next((ln.split(":", 1)[1].strip() for ln in cpu.splitlines() if ln.startswith("model name")), "")
     return dict(value=round(v, 4), unit="MLUPS", cores=th, kind="port",
-                sample=f"oracle (FP64 C restatement of the reference loop, OpenMP over sites, {th} threads), "
-                       f"binary {sample_n}^3 periodic, {sample_steps} steps, same physics as the workload",
+                sample=f"oracle (FP64 C restatement of the reference loop, OpenMP over sites, {th} threads): "
+                       f"the ensemble's first {len(picks)} lattices with edges <= 56 "
+                       f"({', '.join(str(e) for _, e in picks)}), {sample_steps} steps each, same physics and "
+                       "initial state as the workload",
                 seconds=round(dt, 3), cpu_model=model,
                 single_core=dict(value=round(v1, 4), unit="MLUPS", cores=1,
-                                 sample=f"binary {single_n}^3, {single_steps} steps", seconds=round(dt1, 3)))
+                                 sample=f"ternary {single_n}^3 lattice of the ensemble, {single_steps} steps",
+                                 seconds=round(dt1, 3)))
+
+
+PHASE_KERNELS = {"persistent": dict(moments="k_batch_moments", dipole="k_batch_dipole", collide="k_batch_collide"),
+                 "collide-moments": dict(moments="k_batch_edge_moments", dipole="k_batch_dipole",
+                                         collide="k_batch_cm"),
+                 "sequential": dict(moments="k_moments", dipole="k_dipole", collide="k_collide")}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-sample-size", type=int, default=96)
-    ap.add_argument("--cpu-sample-steps", type=int, default=20)
-    ap.add_argument("--cpu-single-size", type=int, default=64)
-    ap.add_argument("--cpu-single-steps", type=int, default=12)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=300, help="timed steps of the ensemble (the headline)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--ensemble-fp64-steps", type=int, default=20)
-    ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--ensemble-accuracy-steps", type=int, default=1000,
-                    help="FP32 against FP64 ensemble after this many steps (0: skip)")
-    ap.add_argument("--no-floor", action="store_true")
+    ap.add_argument("--size", type=int, default=256, help="edge of the binary spinodal lattice")
+    ap.add_argument("--binary-steps", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="FP32 against FP64 after this many steps, binary lattice and ensemble (0: skip)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy runs of the ternary and Lees-Edwards configurations")
     ap.add_argument("--config-steps-scale", type=float, default=1.0)
+    ap.add_argument("--cpu-sample-lattices", type=int, default=8)
+    ap.add_argument("--cpu-sample-steps", type=int, default=10)
+    ap.add_argument("--cpu-single-size", type=int, default=32)
+    ap.add_argument("--cpu-single-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
+                    help="skip the rocprofv3 --pmc child runs that measure HBM traffic")
     ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--ternary-probe", type=int, default=0, help=argparse.SUPPRESS)
-    ap.add_argument("--ensemble-probe", type=int, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--ensemble-probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--le-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -445,7 +678,10 @@ def main():
         ternary_probe(args.ternary_probe)
         return
     if args.ensemble_probe is not None:
-        ensemble_probe(args.ensemble_probe)
+        ensemble_probe(json.loads(args.ensemble_probe))
+        return
+    if args.le_probe is not None:
+        le_probe(args.le_probe)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -454,37 +690,56 @@ def main():
 
     from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64
 
+    # ---- the headline: the FP32-batched ternary ensemble -----------------
+    print(f"[bench] ensemble: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    ens = run_ensemble(args.steps, args.warmup, args.ensemble_fp64_steps)
+    r = ens["fp32"]
+    sites, B = ens["sites"], ENS_BYTES
+    plan = batch_plan(r["plan"])
+    ev = r["event_ms_per_step"]
+    achieved = B * sites / (ev * 1e-3) / 1e9
+    names = PHASE_KERNELS[plan["kind"]]
+    dom = max(r["phases_ms"], key=r["phases_ms"].get)
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel=(f"one graph-replayed step of the ensemble on the tuner's plan ({r['plan']}): "
+                            f"{', '.join(names.values())}"),
+                    algorithmic_bytes_per_launch=B * sites,
+                    algorithmic_bytes_note=f"{B} B per site update (3 x 19 FP32 populations and the 3-vector "
+                                           f"dipole field, read and written) x {sites} sites per step",
+                    mean_launch_ms=round(ev, 5),
+                    launch_timing="HIP events on the batch's stream around the timed steps / steps",
+                    dominant_kernel=dict(name=names[dom], ms_per_step=r["phases_ms"][dom],
+                                         timing="HIP events around each launch over 3 steps"))
+    if not args.no_traffic:
+        tr, why = measure_ensemble_traffic(plan)
+        if tr:
+            roofline["traffic"] = round(tr["bytes_per_step"])
+            roofline["traffic_detail"] = tr
+            dk = tr["kernels"].get(names[dom])
+            if dk:
+                roofline["dominant_kernel"]["traffic_bytes_per_step"] = dk["read"] + dk["write"]
+                roofline["dominant_kernel"]["traffic_rate_gbs"] = round(
+                    (dk["read"] + dk["write"]) / (r["phases_ms"][dom] * 1e-3) / 1e9, 1)
+            print(f"[bench] ensemble traffic {tr['bytes_per_step'] / 1e9:.3f} GB per step "
+                  f"(algorithmic {B * sites / 1e9:.3f}) {tr['kernels']}", file=sys.stderr, flush=True)
+        else:
+            roofline["traffic_note"] = why
+            print(f"[bench] ensemble traffic not measured: {why}", file=sys.stderr, flush=True)
+    print("[bench] HBM ceiling probe", file=sys.stderr, flush=True)
+    ceiling = hbm_ceiling()
+    ceiling["step_algorithmic_frac_of_copy"] = round(achieved / ceiling["copy"], 4)
+    if roofline["traffic"]:
+        ceiling["step_traffic_frac_of_copy"] = round(roofline["traffic"] / (ev * 1e-3) / 1e9 / ceiling["copy"], 4)
+
+    # ---- the binary spinodal lattice: FP32 deviation and FP64 plain ------
     n, ncomp = args.size, 2
-    print(f"[bench] FP64 plain {n}^3 binary: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    r64 = run_form(n, ncomp, LB3D_FP64, args.steps, args.warmup, args.profile_steps)
-    print(f"[bench] FP64 {r64['mlups']:.1f} MLUPS {r64['kernels']}", file=sys.stderr, flush=True)
-    print(f"[bench] FP32 deviation {n}^3 binary", file=sys.stderr, flush=True)
-    r32 = run_form(n, ncomp, LB3D_FP32_DEV, args.steps, args.warmup, args.profile_steps)
-    print(f"[bench] FP32 {r32['mlups']:.1f} MLUPS {r32['kernels']}", file=sys.stderr, flush=True)
-
-    linf, l2 = errs(r32["rho"], r64["rho"])
-    vlinf, vl2 = errs(r32["u"], r64["u"])
-    acc = dict(steps=args.warmup + args.steps, density_linf_rel=linf, density_l2_rel=l2,
-               velocity_linf_rel=vlinf, velocity_l2_rel=vl2,
-               density_std_rel=float(abs(r32["rho"][0].std() - r64["rho"][0].std()) / r64["rho"][0].std()),
-               mass_rel=float(max(abs(a - b) / abs(b) for a, b in zip(r32["mass"][:ncomp], r64["mass"][:ncomp]))))
-    del r32["rho"], r32["u"]
-    if not args.no_floor:
-        # intrinsic floor of the unstable flow: FP64 against FP64 whose initial
-        # mean density differs by 2^-51 (about 2 ulp)
-        print("[bench] FP64 floor run (perturbed initial state)", file=sys.stderr, flush=True)
-        rf = run_form(n, ncomp, LB3D_FP64, args.steps, args.warmup, 1, rho_shift=2.0 ** -51)
-        flinf, fl2 = errs(rf["rho"], r64["rho"])
-        acc["fp64_perturbed_floor"] = dict(density_linf_rel=flinf, density_l2_rel=fl2,
-                                           velocity_linf_rel=errs(rf["u"], r64["u"])[0],
-                                           perturbation="initial mean density + 2^-51")
-        acc["density_linf_over_floor"] = linf / max(flinf, 1e-300)
-        del rf
-    del r64["rho"], r64["u"]
-
+    print(f"[bench] binary {n}^3: FP64 plain and FP32 deviation, {args.binary_steps} steps", file=sys.stderr,
+          flush=True)
+    r64 = run_form(n, ncomp, LB3D_FP64, args.binary_steps, 10, args.profile_steps)
+    r32 = run_form(n, ncomp, LB3D_FP32_DEV, args.binary_steps, 10, args.profile_steps)
     ns = n ** 3
-    B32 = bytes_per_site(ncomp, False, 4)
-    B64 = bytes_per_site(ncomp, False, 8)
+    B32, B64 = bytes_per_site(ncomp, False, 4), bytes_per_site(ncomp, False, 8)
     kname = "fused" if "fused" in r32["kernels"] else "collide"
     if kname == "fused":
         # the one-pass plan launches exactly one kernel per periodic step:
@@ -492,87 +747,87 @@ def main():
         kc, how = r32["event_ms_per_step"], "HIP events on the library stream around the timed steps / steps"
     else:
         kc, how = r32["kernels"][kname], "HIP events on the library stream around each launch"
-    achieved = B32 * ns / (kc * 1e-3) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=f"k_{'fused2' if kname == 'fused' else 'collide'} (FP32 deviation), plan {r32['plan']}",
-                    algorithmic_bytes_per_launch=B32 * ns, mean_launch_ms=round(kc, 5), launch_timing=how,
-                    step_frac=round(r32["mlups"] * 1e6 * B32 / (HBM_PEAK_GBS * 1e9), 4))
+    bach = B32 * ns / (kc * 1e-3) / 1e9
+    broof = dict(bound="hbm", achieved=round(bach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                 frac=round(bach / HBM_PEAK_GBS, 4), traffic=None,
+                 kernel=f"k_{'fused2' if kname == 'fused' else 'collide'} (FP32 deviation), plan {r32['plan']}",
+                 algorithmic_bytes_per_launch=B32 * ns, mean_launch_ms=round(kc, 5), launch_timing=how)
     if not args.no_traffic:
         key = "k_fused2" if kname == "fused" else "k_collide"
-        tr, why = measure_traffic(n, parse_plan(r32["plan"]), key)
-        if tr:
-            roofline["traffic"] = round(tr["bytes_per_launch"])
-            roofline["traffic_detail"] = tr
-            print(f"[bench] traffic {tr['bytes_per_launch'] / 1e9:.3f} GB per launch "
-                  f"(algorithmic {B32 * ns / 1e9:.3f})", file=sys.stderr, flush=True)
+        btr, why = measure_traffic(n, parse_plan(r32["plan"]), key)
+        if btr:
+            broof["traffic"] = round(btr["bytes_per_launch"])
+            broof["traffic_detail"] = btr
         else:
-            roofline["traffic_note"] = why
-            print(f"[bench] traffic not measured: {why}", file=sys.stderr, flush=True)
-    print("[bench] HBM ceiling probe", file=sys.stderr, flush=True)
-    ceiling = hbm_ceiling()
-    # the dominant kernel against what a plain copy reaches on this card
-    ceiling["kernel_algorithmic_frac_of_copy"] = round(achieved / ceiling["copy"], 4)
-    if roofline["traffic"]:
-        ceiling["kernel_traffic_frac_of_copy"] = round(roofline["traffic"] / (kc * 1e-3) / 1e9 / ceiling["copy"], 4)
-    print(f"[bench] ceiling {ceiling}", file=sys.stderr, flush=True)
-    fp64 = dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
-                step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
-                kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()}, plan=r64["plan"])
-    ens = None
-    if not args.no_ensemble:
-        ens = run_ensemble(args.ensemble_steps, 5, args.ensemble_fp64_steps)
-        # the step's algorithmic rate against what a plain copy reaches on this card
-        ens["step_frac_of_copy"] = round(ens["value"] * 1e6 * ens["bytes_per_site"] / 1e9 / ceiling["copy"], 4)
-        if args.ensemble_accuracy_steps > 0:
-            ens["accuracy_fp32_vs_fp64"] = ensemble_accuracy(args.ensemble_accuracy_steps)
-    configs = None
+            broof["traffic_note"] = why
+    binary = dict(value=round(r32["mlups"], 1), unit="MLUPS", ms_per_step=round(r32["ms_per_step"], 4),
+                  steps=args.binary_steps, lattice=[n, n, n], bytes_per_site=B32,
+                  workload=f"binary Shan-Chen spinodal D3Q19 {n}^3 periodic, g_br=0.08, tau=1, bdist=2, "
+                           "rho=0.7+-0.01 (seed 1), FP32 deviation storage",
+                  roofline=broof, kernels_ms={k: round(v, 5) for k, v in r32["kernels"].items()},
+                  fp64_plain=dict(value=round(r64["mlups"], 1), unit="MLUPS", ms_per_step=round(r64["ms_per_step"], 4),
+                                  step_frac=round(r64["mlups"] * 1e6 * B64 / (HBM_PEAK_GBS * 1e9), 4),
+                                  kernels_ms={k: round(v, 5) for k, v in r64["kernels"].items()}, plan=r64["plan"]),
+                  speedup_fp32_vs_fp64=round(r32["mlups"] / r64["mlups"], 3))
+    print(f"[bench] binary {n}^3: FP32 {binary['value']} MLUPS, FP64 {binary['fp64_plain']['value']} MLUPS",
+          file=sys.stderr, flush=True)
+
+    # ---- FP32 against FP64 per configuration ------------------------------
+    accuracy = {}
+    if args.accuracy_steps > 0:
+        st = args.accuracy_steps
+        cps = [c for c in (10, 30, 100, 300, 1000, 2000, 5000) if c < st]
+        accuracy["binary_256"] = config_accuracy(
+            f"binary {n}^3", (n, n, n), st, dict(ncomp=2, g_br=0.08, bdist=2), (0.01, 0.01, 0.0), cps)
+        accuracy["batched_ternary_ensemble"] = ensemble_accuracy(st, [c for c in (30, 100, 300) if c < st])
     if not args.no_configs:
         sc = args.config_steps_scale
-        configs = {
-            "ternary_128": config_accuracy(
-                "ternary 128^3", (128, 128, 128), max(1, int(2000 * sc)),
-                dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2),
-                (0.01, 0.01, 0.0)),
-            "lees_edwards_256x64x64": config_accuracy(
-                "Lees-Edwards 256x64x64", (256, 64, 64), max(1, int(5000 * sc)),
-                dict(ncomp=2, g_br=0.08, bdist=2, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.05),
-                (0.01, 0.01, 0.0)),
-        }
+        s2, s3 = max(1, int(2000 * sc)), max(1, int(5000 * sc))
+        accuracy["ternary_128"] = config_accuracy(
+            "ternary 128^3", (128, 128, 128), s2,
+            dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2),
+            (0.01, 0.01, 0.0), [c for c in (10, 30, 100, 300, 1000) if c < s2])
+        accuracy["lees_edwards_256x64x64"] = config_accuracy(
+            "Lees-Edwards 256x64x64", (256, 64, 64), s3,
+            dict(ncomp=2, g_br=0.08, bdist=2, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.05),
+            (0.01, 0.01, 0.0), [c for c in (10, 100, 300, 1000, 3000) if c < s3])
     cpu = None
     if not args.no_cpu_baseline:
-        print(f"[bench] CPU baseline oracle {args.cpu_sample_size}^3 x{args.cpu_sample_steps} (all threads), "
-              f"{args.cpu_single_size}^3 x{args.cpu_single_steps} (one core)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_sample_size, args.cpu_sample_steps, args.cpu_single_size,
+        print(f"[bench] CPU baseline: oracle on {args.cpu_sample_lattices} ensemble lattices x "
+              f"{args.cpu_sample_steps} steps (all threads), {args.cpu_single_size}^3 x {args.cpu_single_steps} "
+              "(one core)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_sample_lattices, args.cpu_sample_steps, args.cpu_single_size,
                            args.cpu_single_steps)
+    fp64s = ens["fp64_sequential"]
     line = {
         "metric": METRIC,
-        "value": round(r32["mlups"], 1),
+        "value": round(r["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r32["ms_per_step"], 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"binary Shan-Chen spinodal D3Q19 {n}^3 periodic, g_br=0.08, tau=1, bdist=2, "
-                               "rho=0.7+-0.01 (seed 1), FP32 deviation storage",
-                   "lattice": [n, n, n], "components": ncomp, "bytes_per_site": B32},
+        "config": {"workload": "batched ternary ensemble: 48 periodic D3Q19 lattices, cube edges uniform in "
+                               "{16,...,96} (seed 1), g_br uniform in [0.06,0.12], g_bs=-0.006, g_ss=-0.003, "
+                               "tau=tau_d=1, beta=10, bdist=2, rho=0.7+-0.01, dipoles on the sphere (seeds 1000+k), "
+                               "FP32 deviation storage, one batch, graph-replayed steps, tuned plan",
+                   "lattices": ens["lattices"], "sites": sites, "edges": ens["edges"], "components": 3,
+                   "bytes_per_site": B, "plan": r["plan"]},
         "roofline": roofline,
         "hbm_ceiling": ceiling,
         "cpu_baseline": cpu,
-        "fp64_plain": fp64,
-        "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
-        "accuracy_fp32_vs_fp64": acc,
-        "kernels_ms": {k: round(v, 5) for k, v in r32["kernels"].items()},
-        "kernels_ms_note": (f"HIP events around each launch over {args.profile_steps} steps launched one by one "
-                            "after the timed loop (not graph-replayed, with an event pair per launch); "
-                            "roofline.mean_launch_ms is the timed loop's"),
-        "batched_ternary_ensemble": ens,
-        "config_accuracy": configs,
+        "fp64_plain": dict(fp64s, workload="the same 48 lattices run one after another on the plain FP64 path"),
+        "speedup_fp32_batched_vs_fp64_plain": round(r["mlups"] / fp64s["value"], 3),
+        "phases_ms": r["phases_ms"],
+        "phases_note": ("HIP events around each launch over 3 steps launched one by one after the timed loop "
+                        "(not graph-replayed); roofline.mean_launch_ms is the timed loop's per-step time"),
+        "binary_256": binary,
+        "accuracy_fp32_vs_fp64": accuracy,
     }
     print(json.dumps(line), flush=True)
 

@@ -271,6 +271,9 @@ int lb3d_gpu_batch_upload_sites(lb3d_batch *b, int32_t k, const double *whole_N,
                                 int32_t halo_extent, int32_t site_doubles);
 /* nsteps timesteps of every lattice. */
 int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps);
+/* lb3d_gpu_batch_run_steps between HIP events on the batch's stream; *ms is
+ * the elapsed time (as lb3d_gpu_run_steps_timed). */
+int lb3d_gpu_batch_run_steps_timed(lb3d_batch *b, int32_t nsteps, double *ms);
 int lb3d_gpu_batch_get_nt(lb3d_batch *b, int32_t *nt);
 int lb3d_gpu_batch_synchronize(lb3d_batch *b);
 /* The full sanity check of lattice k (as lb3d_gpu_sanity). */

@@ -445,6 +445,36 @@ int lb3d_gpu_batch_run_steps(lb3d_batch *b, int32_t nsteps)
     return LB3D_OK;
 }
 
+int lb3d_gpu_batch_run_steps_timed(lb3d_batch *b, int32_t nsteps, double *ms)
+{
+    clear_error();
+    if (!b || !ms || nsteps < 1) return set_error(LB3D_EINVAL, "bad argument");
+    // prepared before the first event: the graph, and the moments the
+    // collide-moments plan carries
+    int rc = ensure_graph(b);
+    if (rc) return rc;
+    if (b->cm_cols && !b->aux_ok) prime_aux(b);
+    hipEvent_t e[2] = {nullptr, nullptr};
+    struct Ev {
+        hipEvent_t *e;
+        ~Ev()
+        {
+            for (int j = 0; j < 2; j++)
+                if (e[j]) (void)hipEventDestroy(e[j]);
+        }
+    } guard{e};
+    BTRY(hipEventCreate(&e[0]));
+    BTRY(hipEventCreate(&e[1]));
+    BTRY(hipEventRecord(e[0], b->stream));
+    if ((rc = lb3d_gpu_batch_run_steps(b, nsteps))) return rc;
+    BTRY(hipEventRecord(e[1], b->stream));
+    BTRY(hipEventSynchronize(e[1]));
+    float t = 0.f;
+    BTRY(hipEventElapsedTime(&t, e[0], e[1]));
+    *ms = t;
+    return LB3D_OK;
+}
+
 int lb3d_gpu_batch_synchronize(lb3d_batch *b)
 {
     clear_error();

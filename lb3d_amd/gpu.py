@@ -42,6 +42,7 @@ EXPORTS = [
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
     "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_set_groups", "lb3d_gpu_batch_autotune",
     "lb3d_gpu_batch_profile_steps", "lb3d_gpu_batch_set_collide_moments", "lb3d_gpu_step_noalloc",
+    "lb3d_gpu_batch_run_steps_timed",
 ]
 
 
@@ -134,6 +135,7 @@ def lib():
         L.lb3d_gpu_batch_init_noise.argtypes = [vp, ip, ctypes.c_uint64, dp, dp, ip]
         L.lb3d_gpu_batch_upload_sites.argtypes = [vp, ip, dp, ip, ip]
         L.lb3d_gpu_batch_run_steps.argtypes = [vp, ip]
+        L.lb3d_gpu_batch_run_steps_timed.argtypes = [vp, ip, dp]
         L.lb3d_gpu_batch_get_nt.argtypes = [vp, ctypes.POINTER(ip)]
         L.lb3d_gpu_batch_synchronize.argtypes = [vp]
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
@@ -429,6 +431,12 @@ class Batch:
 
     def time_loop(self, nsteps):
         check(lib().lb3d_gpu_batch_run_steps(self._h, int(nsteps)), "batch_run_steps")
+
+    def time_loop_timed(self, nsteps):
+        """time_loop between HIP events on the batch's stream; returns ms."""
+        ms = ctypes.c_double()
+        check(lib().lb3d_gpu_batch_run_steps_timed(self._h, int(nsteps), ctypes.byref(ms)), "batch_run_steps_timed")
+        return ms.value
 
     def synchronize(self):
         check(lib().lb3d_gpu_batch_synchronize(self._h), "batch_synchronize")

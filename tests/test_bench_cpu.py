@@ -25,3 +25,16 @@ def test_bytes_per_site_model():
     assert bench.bytes_per_site(2, False, 8) == 608
     assert bench.bytes_per_site(3, True, 4) == 480
     assert bench.bytes_per_site(3, True, 8) == 960
+
+
+def test_batch_plan_lines_parse():
+    """The batch tuner's report lines become the plan the traffic probe re-applies."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    p = bench.batch_plan("chosen: persistent nwg=51104 xg=16 groups=18 (192 MiB) 2.2412 ms")
+    assert p == dict(kind="persistent", nwg=51104, xg=16, group_mib=192)
+    p = bench.batch_plan("chosen: collide-moments cols=62 threads=512 lz=8 xg=16 2.5400 ms")
+    assert p == dict(kind="collide-moments", cols=62, threads=512, lz=8, xg=16)
+    assert bench.batch_plan("chosen: sequential nwg=0 xg=0 groups=0 (0 MiB) 5.1 ms")["kind"] == "sequential"
+    assert bench.ENS_BYTES == bench.bytes_per_site(3, True, 4)
