@@ -223,6 +223,16 @@ int lb3d_gpu_get_momentum(lb3d_ctx *ctx, double *out);
  * needs a completed timestep.  Synchronises. */
 int lb3d_gpu_le_halo(lb3d_ctx *ctx, double *out);
 
+/* The arr observable of the reference's dumps (dump_arrows,
+ * code/lb3d_io_dump_data.F90:1308-1460) in FP64: per site
+ * (sum_c amass_c (P_c - F_c rho_c / 2)) / max(10.e-9, sum_c amass_c rho_c),
+ * less g_accn/2 inside the body-force box, with the Shan-Chen forces F_c of
+ * the state (from the densities of the last advection and the dipole field,
+ * periodic in x as the reference's dumps see it) and rho_c, P_c summed from
+ * the populations; out[3*(x + nx*(y + ny*z)) + a] (the order of the dump).
+ * Needs a completed timestep (LB3D_ESTATE otherwise).  Synchronises. */
+int lb3d_gpu_get_arrows(lb3d_ctx *ctx, double *out);
+
 /* Wait for all work queued on the context's stream. */
 int lb3d_gpu_synchronize(lb3d_ctx *ctx);
 

@@ -451,6 +451,22 @@ int lb3d_gpu_get_populations(lb3d_ctx *c, double *out)
     return LB3D_OK;
 }
 
+// The one-pass plans keep no moments: rebuild those of the last advection
+// from the previous post-collision populations f[1-cur] and the dipole field
+// that step read (db_alt).
+static int rebuild_aux(lb3d_ctx *c)
+{
+    if (c->aux_valid || !c->aux_from_prev) return LB3D_OK;
+    DevBufs prev = c->b;
+    prev.cur = 1 - c->b.cur;
+    if (c->p.ncomp == 3) prev.db = c->b.db_alt;
+    c->L->moments(prev, c->stream);
+    HIP_TRY(hipGetLastError());
+    c->aux_valid = true;
+    c->aux_from_prev = false;
+    return LB3D_OK;
+}
+
 int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
 {
     g_err.clear();
@@ -479,18 +495,7 @@ int lb3d_gpu_download_sites(lb3d_ctx *c, double *N, int32_t h, int32_t sd)
         }
         return LB3D_OK;
     };
-    if (!c->aux_valid && c->aux_from_prev) {
-        // the one-pass plan keeps no moments: recompute those of the last
-        // advection from the previous post-collision populations f[1-cur]
-        // (ternary: with the dipole field of that step, now in db_alt)
-        DevBufs prev = c->b;
-        prev.cur = 1 - c->b.cur;
-        if (C == 3) prev.db = c->b.db_alt;
-        c->L->moments(prev, c->stream);
-        HIP_TRY(hipGetLastError());
-        c->aux_valid = true;
-        c->aux_from_prev = false;
-    }
+    if ((rc = rebuild_aux(c))) return rc;
     if (c->aux_valid && (rc = fetch(c->b.aux, c->aux_elems, aux))) return rc;
     if (C == 3 && (rc = fetch(c->b.db, 3 * ns, db))) return rc;
     std::vector<uint8_t> rock;
@@ -1009,6 +1014,88 @@ int lb3d_gpu_get_momentum(lb3d_ctx *c, double *out)
             }
         }
     }
+    return LB3D_OK;
+}
+
+// The reference's arr observable (dump_arrows, code/lb3d_io_dump_data.F90:
+// 1308-1460): per site the momentum of every component less half its
+// Shan-Chen force times its density, weighted by amass, over the
+// amass-weighted density (max with 10.e-9 in single precision, as the
+// reference's real() and single literal give), less half the body-force
+// acceleration inside its box.  The forces come from the device (the
+// collide's own force code, from the moments of the last advection and the
+// dipole field); the sums are FP64 in the reference's order.
+int lb3d_gpu_get_arrows(lb3d_ctx *c, double *out)
+{
+    g_err.clear();
+    if (!c || !out) return fail(LB3D_EINVAL, "NULL argument");
+    if (c->state != IDLE) return fail(LB3D_ESTATE, "get_arrows needs a completed timestep");
+    int rc = rebuild_aux(c);
+    if (rc) return rc;
+    if (!c->aux_valid)
+        return fail(LB3D_ESTATE, "get_arrows needs the moments of an advection: run a timestep first");
+    const int C = c->p.ncomp, nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const size_t ns = (size_t)nx * ny * nz, rs = c->L->real_bytes();
+    std::vector<double> pops((size_t)C * 19 * ns), F((size_t)C * 3 * ns);
+    if ((rc = lb3d_gpu_get_populations(c, pops.data()))) return rc;
+    {
+        // a temporary device buffer of the force field (not a hot path)
+        void *d = nullptr;
+        HIP_TRY(hipMalloc(&d, F.size() * rs));
+        c->L->forces(c->b, d, c->stream);
+        std::vector<unsigned char> h(F.size() * rs);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, h.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(d);
+        if (e != hipSuccess) return fail(LB3D_EHIP, std::string("forces: ") + hipGetErrorString(e));
+        for (size_t q = 0; q < F.size(); q++)
+            F[q] = rs == 8 ? ((const double *)h.data())[q] : (double)((const float *)h.data())[q];
+    }
+    std::vector<uint8_t> rock;
+    if (c->b.rock) {
+        rock.resize(ns);
+        HIP_TRY(hipMemcpyAsync(rock.data(), c->b.rock, ns, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    const lb3d_params &p = c->p;
+    const bool body = p.g_accn[0] != 0.0 || p.g_accn[1] != 0.0 || p.g_accn[2] != 0.0;
+    const int ext[3] = {nx, ny, nz};
+    int bmax[3];
+    for (int k = 0; k < 3; k++) bmax[k] = p.g_accn_max[k] == 0 ? ext[k] : p.g_accn_max[k];
+    const int *cv[3] = {host::CX, host::CY, host::CZ};
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const size_t s = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
+                const bool fluid = rock.empty() || !rock[s];
+                double rho[3] = {0, 0, 0}, mom[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int k = 0; k < C; k++)
+                    for (int i = 0; i < 19; i++) {
+                        const double v = pops[((size_t)k * 19 + i) * ns + s] * (double)host::GD[i];
+                        rho[k] = rho[k] + v;
+                        for (int a = 0; a < 3; a++) mom[k][a] = mom[k][a] + v * (double)cv[a][i];
+                    }
+                double den = rho[0] * p.amass[0];
+                for (int k = 1; k < C; k++) den = den + rho[k] * p.amass[k];
+                const float denf = (float)den;
+                const double dd = (double)(denf > 10.e-9f ? denf : 10.e-9f);
+                const bool inbox = body && fluid && z + 1 >= p.g_accn_min[2] && z + 1 <= bmax[2] &&
+                                   x + 1 >= p.g_accn_min[0] && x + 1 <= bmax[0] && y + 1 >= p.g_accn_min[1] &&
+                                   y + 1 <= bmax[1];
+                for (int a = 0; a < 3; a++) {
+                    double v = 0.0;
+                    for (int k = 0; k < C; k++) {
+                        const double fa = fluid ? F[((size_t)k * 3 + a) * ns + s] : 0.0;
+                        const double A = mom[k][a] * p.amass[k];
+                        const double B = ((fa / 2.0) * rho[k]) * p.amass[k];
+                        v = k == 0 ? A - B : (v + A) - B;
+                    }
+                    v = v / dd;
+                    if (inbox) v = v - p.g_accn[a] / 2.0;
+                    out[3 * s + a] = v;
+                }
+            }
     return LB3D_OK;
 }
 

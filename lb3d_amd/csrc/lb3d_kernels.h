@@ -702,6 +702,36 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
     }
 }
 
+// The Shan-Chen force field of the state (the arr observable of the
+// reference's dump_arrows, code/lb3d_io_dump_data.F90:1308-1460): the force
+// of every site from the aux field of the last advection and the dipole
+// field, with periodic x (the reference's dumps see periodic halos also under
+// Lees-Edwards); rock sites feel none.  out[(c*3 + k)*ns + s].
+template <typename R, bool Dev, int C, bool Rock>
+__global__ __launch_bounds__(256) void k_forces(Phys<R> P, const R *__restrict__ aux, const R *__restrict__ db,
+                                                const uint8_t *__restrict__ rock, R *__restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.ns) return;
+    int x, y, z;
+    site_xyz(s, P.nx, P.ny, x, y, z);
+    R F[C][3];
+    if constexpr (Rock) {
+        if (rock[s]) {
+#pragma unroll
+            for (int c = 0; c < C; c++) F[c][0] = F[c][1] = F[c][2] = R(0);
+        } else {
+            sc_forces_wall<R, Dev>(P, aux, rock, x, y, z, s, F);
+        }
+    } else {
+        sc_forces<R, Dev, C, false>(P, aux, nullptr, db, x, y, z, s, F);
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) st_plane(out + (size_t)(c * 3 + k) * P.ns, s, F[c][k]);
+}
+
 // --------------------------------------------------------------------------
 // Collision of one site from its streamed populations n and Shan-Chen forces
 // F: moments, body force, BGK relaxation towards rho*boltz(u_tilde, du),

@@ -275,6 +275,16 @@ class LauncherT : public Launcher {
         else
             k_moments<R, Dev, 2, false><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg, nullptr);
     }
+    void forces(const DevBufs &b, void *out, hipStream_t s) override
+    {
+        if (C == 3)
+            k_forces<R, Dev, 3, false><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, (const R *)b.db, nullptr,
+                                                                  (R *)out);
+        else if (b.rock)
+            k_forces<R, Dev, 2, true><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, nullptr, b.rock, (R *)out);
+        else
+            k_forces<R, Dev, 2, false><<<blocks(), BLOCK, 0, s>>>(P, (const R *)b.aux, nullptr, nullptr, (R *)out);
+    }
     void dipole(const DevBufs &b, hipStream_t s) override
     {
         // the relaxed field goes to db_alt (the collide reads it there)

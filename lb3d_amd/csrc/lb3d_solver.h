@@ -39,6 +39,9 @@ class Launcher {
     virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
     virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
     virtual void le_faces(const DevBufs &b, bool post, hipStream_t s) = 0;
+    // the Shan-Chen force field from b.aux and b.db into out[(c*3 + k)*ns + s]
+    // (working type R), periodic in x
+    virtual void forces(const DevBufs &b, void *out, hipStream_t s) = 0;
     // the whole binary step in one kernel (f[cur] -> f[1-cur]); false if the
     // tile shape has no instantiation or the form is not binary
     // stage: where plane k's populations wait for the collide: 0 registers,

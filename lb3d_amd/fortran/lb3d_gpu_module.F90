@@ -10,6 +10,7 @@ module lb3d_gpu_module
 
   integer(c_int32_t), parameter, public :: LB3D_ABI_VERSION = 1
   integer(c_int32_t), parameter, public :: LB3D_OK = 0
+  integer(c_int32_t), parameter, public :: LB3D_EINVAL = 1, LB3D_EHIP = 2, LB3D_ESTATE = 3, LB3D_ENOMEM = 4
   integer(c_int32_t), parameter, public :: LB3D_FP64 = 0, LB3D_FP32_DEV = 1
   integer(c_int32_t), parameter, public :: LB3D_BC_PERIODIC = 0, LB3D_BC_LEES_EDWARDS = 1
 
@@ -50,7 +51,7 @@ module lb3d_gpu_module
 
   public :: lb3d_gpu_abi_version, lb3d_gpu_create, lb3d_gpu_destroy
   public :: lb3d_gpu_sanity
-  public :: lb3d_gpu_upload_sites, lb3d_gpu_download_sites, lb3d_gpu_init_noise
+  public :: lb3d_gpu_upload_sites, lb3d_gpu_download_sites, lb3d_gpu_init_noise, lb3d_gpu_get_arrows
   public :: lb3d_gpu_bc_init, lb3d_gpu_force_apply, lb3d_gpu_bc_apply
   public :: lb3d_gpu_advection, lb3d_gpu_collision, lb3d_gpu_run_steps
   public :: lb3d_gpu_get_nt, lb3d_gpu_diagnostics, lb3d_gpu_le_state
@@ -98,6 +99,13 @@ module lb3d_gpu_module
        integer(c_int32_t), value :: halo, sd
        integer(c_int32_t) :: rc
      end function lb3d_gpu_download_sites
+
+     function lb3d_gpu_get_arrows(ctx, out) bind(C, name="lb3d_gpu_get_arrows") result(rc)
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: out(*)
+       integer(c_int32_t) :: rc
+     end function lb3d_gpu_get_arrows
 
      function lb3d_gpu_init_noise(ctx, seed, rho_mean, amp, dipoles) &
           bind(C, name="lb3d_gpu_init_noise") result(rc)

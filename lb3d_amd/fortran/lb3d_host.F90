@@ -410,8 +410,10 @@ contains
     integer :: x, y, z, c, s
     logical :: any_dump
     any_dump = dump_now(sci_od, n_sci_od) .or. dump_now(sci_wd, n_sci_wd) .or. &
-         dump_now(sci_sur, n_sci_sur) .or. dump_now(sci_dir, n_sci_dir) .or. dump_now(sci_vel, n_sci_vel)
+         dump_now(sci_sur, n_sci_sur) .or. dump_now(sci_dir, n_sci_dir) .or. dump_now(sci_vel, n_sci_vel) .or. &
+         dump_now(sci_arrows, n_sci_arrows)
     if (.not. any_dump) return
+    if (dump_now(sci_arrows, n_sci_arrows)) call dump_arrows()
     call check(lb3d_gpu_download_sites(ctx, whole_N, 1_c_int32_t, int(sd, c_int32_t)), 'download_sites')
     amass = [amass_r, amass_b, amass_s]
     omega = 1.0_rk / [tau_r, tau_b, tau_s]
@@ -474,6 +476,24 @@ contains
        call dump_scalar_xdr('vel', sc)
     end if
   end subroutine lb3d_io_write_data
+
+  !> arr (dump_arrows, code/lb3d_io_dump_data.F90:1308-1460): the velocity
+  !> with half the Shan-Chen force taken off, from the library
+  !> (lb3d_gpu_get_arrows).  Its forces need the densities of an advection,
+  !> so before the first timestep of a run there is nothing to dump: that is
+  !> logged, not silently skipped.
+  subroutine dump_arrows()
+    real(rk), allocatable :: vec(:, :, :, :)
+    integer(c_int32_t) :: rc
+    allocate(vec(3, nx, ny, nz))
+    rc = lb3d_gpu_get_arrows(ctx, vec)
+    if (rc == LB3D_ESTATE) then
+       call log_msg('WARNING: arr not dumped at this timestep: no advection has run yet')
+       return
+    end if
+    call check(rc, 'get_arrows')
+    call dump_vector_xdr('arr', vec)
+  end subroutine dump_arrows
 
   subroutine lb3d_io_write_checkpoint()
     if (n_checkpoint <= 0 .or. nt == tstart) return
@@ -637,6 +657,7 @@ contains
     call nearer(sci_sur, n_sci_sur)
     call nearer(sci_dir, n_sci_dir)
     call nearer(sci_vel, n_sci_vel)
+    call nearer(sci_arrows, n_sci_arrows)
     call nearer(.true., n_checkpoint)
     call nearer(.true., n_sanity_check)
   contains

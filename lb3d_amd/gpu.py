@@ -42,7 +42,7 @@ EXPORTS = [
     "lb3d_gpu_batch_synchronize", "lb3d_gpu_batch_get_populations", "lb3d_gpu_batch_diagnostics",
     "lb3d_gpu_batch_set_plan", "lb3d_gpu_batch_set_groups", "lb3d_gpu_batch_autotune",
     "lb3d_gpu_batch_profile_steps", "lb3d_gpu_batch_set_collide_moments", "lb3d_gpu_step_noalloc",
-    "lb3d_gpu_batch_run_steps_timed",
+    "lb3d_gpu_batch_run_steps_timed", "lb3d_gpu_get_arrows",
 ]
 
 
@@ -136,6 +136,7 @@ def lib():
         L.lb3d_gpu_batch_upload_sites.argtypes = [vp, ip, dp, ip, ip]
         L.lb3d_gpu_batch_run_steps.argtypes = [vp, ip]
         L.lb3d_gpu_batch_run_steps_timed.argtypes = [vp, ip, dp]
+        L.lb3d_gpu_get_arrows.argtypes = [vp, dp]
         L.lb3d_gpu_batch_get_nt.argtypes = [vp, ctypes.POINTER(ip)]
         L.lb3d_gpu_batch_synchronize.argtypes = [vp]
         L.lb3d_gpu_batch_get_populations.argtypes = [vp, ip, dp]
@@ -254,6 +255,12 @@ class Lattice:
         out = np.empty((self.ncomp, 3, self.nz, self.ny, self.nx))
         check(lib().lb3d_gpu_get_momentum(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
               "get_momentum")
+        return out
+
+    def arrows(self):
+        """The reference's arr observable, shape (nz, ny, nx, 3) in FP64."""
+        out = np.empty((self.nz, self.ny, self.nx, 3))
+        check(lib().lb3d_gpu_get_arrows(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "get_arrows")
         return out
 
     def le_halo(self):

@@ -17,11 +17,11 @@ INPUTS = os.path.join(ROOT, "tests", "golden", "inputs")
 
 pytestmark = pytest.mark.gpu
 
-CASES = {"gyroid": (3, ["od", "wd", "sur", "dir", "vel"]),
-         "leesEdwards": (2, ["od", "wd", "vel"]),
-         "bodyForce": (3, ["od", "wd", "sur", "dir", "vel"]),
-         "wettingWall": (2, ["od", "wd", "vel"]),
-         "zouHe": (3, ["od", "wd", "sur", "dir", "vel"])}
+CASES = {"gyroid": (3, ["od", "wd", "sur", "dir", "vel", "arr"]),
+         "leesEdwards": (2, ["od", "wd", "vel", "arr"]),
+         "bodyForce": (3, ["od", "wd", "sur", "dir", "vel", "arr"]),
+         "wettingWall": (2, ["od", "wd", "vel", "arr"]),
+         "zouHe": (3, ["od", "wd", "sur", "dir", "vel", "arr"])}
 
 
 def write_checkpoint(dirname, name, gold):

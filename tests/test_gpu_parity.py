@@ -197,6 +197,34 @@ def test_reference_golden_tests_on_gpu(name):
     for ob in obs:
         mine = chk.dump(ob).astype(np.float32)
         np.testing.assert_array_equal(mine, gold[f"{name}/{ob}"], err_msg=f"{name}/{ob}")
+    # the library's own arr observable (forces on the device, the reference's
+    # sums on the host) against the oracle's dump routine and the golden
+    arr = g.arrows()
+    np.testing.assert_allclose(arr, chk.dump("arr"), rtol=0, atol=1e-13, err_msg=f"{name}/arr")
+    np.testing.assert_array_equal(arr.astype(np.float32), gold[f"{name}/arr"], err_msg=f"{name}/arr float32")
+
+
+@pytest.mark.parametrize("kw", [dict(ncomp=2, g_br=0.08), dict(ncomp=3, g_br=0.08, g_bs=-0.006, g_ss=-0.003),
+                                dict(ncomp=2, g_br=0.09, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.04)])
+def test_arrows_match_oracle_and_fp32(kw):
+    """arr after 12 steps on a lattice that does not divide into tiles: FP64
+    against the oracle's dump_arrows of the GPU state, the FP32 form within
+    its rounding, and LB3D_ESTATE before the first advection."""
+    dims = (13, 10, 7)
+    o, g = pair(*dims, **kw, amp=(0.02, 0.02, 0.01))
+    with pytest.raises(LB3DError):
+        g.arrows()
+    g.time_loop(12)
+    okw = {k: v for k, v in kw.items() if k not in ("boundary",)}
+    if kw.get("boundary") == LB3D_BC_LEES_EDWARDS:
+        okw["lees_edwards"] = 1
+    chk = Oracle(*dims, **okw)
+    chk.N[...] = g.download_sites()
+    want = chk.dump("arr")
+    np.testing.assert_allclose(g.arrows(), want, rtol=0, atol=1e-13)
+    _, f = pair(*dims, **kw, amp=(0.02, 0.02, 0.01), precision=LB3D_FP32_DEV)
+    f.time_loop(12)
+    np.testing.assert_allclose(f.arrows(), want, rtol=0, atol=2e-6)
 
 
 @pytest.mark.parametrize("itype", [-1, 0, 1])
