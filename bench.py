@@ -316,7 +316,8 @@ def copy_calibration():
 
 
 def counter_sums(outdir, counter):
-    """Sum and count of `counter` over the dispatches of each kernel, from a rocprofv3 database."""
+    """Sum, count and largest value of `counter` over the dispatches of each
+    kernel, from a rocprofv3 database."""
     import glob
     import sqlite3
 
@@ -327,9 +328,10 @@ def counter_sums(outdir, counter):
     acc = {}
     for name, val in con.execute("select kernel_name, value from counters_collection where counter_name=?",
                                  (counter,)):
-        t = acc.setdefault(name, [0.0, 0])
+        t = acc.setdefault(name, [0.0, 0, 0.0])
         t[0] += val
         t[1] += 1
+        t[2] = max(t[2], val)
     con.close()
     return acc
 
@@ -379,12 +381,14 @@ def measure_ensemble_traffic(plan):
         sums, why = run_pmc(counter, ["--ensemble-probe", json.dumps(plan)])
         if sums is None:
             return None, why
-        copy = [v[0] / v[1] for k, v in sums.items() if "lb3d" not in k and "copy" in k.lower()]
+        # the calibration copies are the largest copy dispatches (the batch's
+        # table and work-list uploads are copies too)
+        copy = [v[2] for k, v in sums.items() if "lb3d" not in k and "copy" in k.lower()]
         if not copy:
             return None, f"rocprofv3 --pmc {counter}: no copy kernel among {sorted(sums)[:6]}"
         scale = COPY_BYTES / (copy[0] * 1024.0)
         per = {}
-        for k, (tot, n) in sums.items():
+        for k, (tot, n, _) in sums.items():
             short = k.split("(")[0].split("<")[0].replace("void lb3d::", "").strip()
             if short in step_kernels:
                 per[short] = per.get(short, 0.0) + tot * 1024.0 * scale / PROBE_STEPS

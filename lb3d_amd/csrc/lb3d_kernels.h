@@ -1536,69 +1536,80 @@ __device__ __forceinline__ int wrapz1(int k, int n)
     return m;
 }
 
+// One work-group of 2 * LE_SITES threads serves LE_SITES halo columns
+// (y, z): its first wave builds the x = 0 plane's sites, the second the
+// x = nx+1 plane's, both interpolating at once; the high side's clamp then
+// reads the finished low site from LDS.
+constexpr int LE_SITES = 64;
 template <typename R, bool Dev, bool Post>
-__global__ __launch_bounds__(128) void k_le_faces(Phys<R> P, const LEState *__restrict__ Lp,
-                                                  const R *__restrict__ f,
-                                                  R *__restrict__ face_pc,
-                                                  R *__restrict__ face_aux)
+__global__ __launch_bounds__(2 * LE_SITES) void k_le_faces(Phys<R> P, const LEState *__restrict__ Lp,
+                                                           const R *__restrict__ f, R *__restrict__ face_pc,
+                                                           R *__restrict__ face_aux)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
-    if (t >= plane) return;
+    __shared__ R lo_done[2][19][LE_SITES];  // the corrected low-plane sites
+    const uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+    const int j = threadIdx.x % LE_SITES;
+    const bool low = threadIdx.x < LE_SITES;  // wave-uniform
+    const uint32_t t = blockIdx.x * LE_SITES + j;
+    const bool live = t < plane;
     const LEState L = *Lp;
-    int y = (int)(t % (uint32_t)P.ny), z = (int)(t / (uint32_t)P.ny);  // 0-based
-    int z1 = z + 1;                                                    // 1-based
-    int lelo = L.lelo, bsh = lelo >= 0 ? 0 : -1;
-    R fr = R(L.lefrac), omf = R(1.0) - R(L.lefrac);
-    R lo[2][19], hi[2][19];
-    {
-        int za = wrapz1(z1 + lelo + bsh, P.nz), zb = wrapz1(z1 + lelo + 1 + bsh, P.nz);
+    const int y = live ? (int)(t % (uint32_t)P.ny) : 0, z = live ? (int)(t / (uint32_t)P.ny) : 0;  // 0-based
+    const int z1 = z + 1;                                                                           // 1-based
+    const int lelo = L.lelo, bsh = lelo >= 0 ? 0 : -1;
+    const R fr = R(L.lefrac), omf = R(1.0) - R(L.lefrac);
+    R n[2][19];
+    if (live) {
+        // low (x=0):    (1-f) N(nx, y, z+lelo+b) + f N(nx, y, z+lelo+1+b)
+        // high (x=nx+1): f N(1, y, z-lelo-1-b)   + (1-f) N(1, y, z-lelo-b)
+        const int xs = low ? P.nx - 1 : 0;
+        const int za = low ? wrapz1(z1 + lelo + bsh, P.nz) : wrapz1(z1 - lelo - 1 - bsh, P.nz);
+        const int zb = low ? wrapz1(z1 + lelo + 1 + bsh, P.nz) : wrapz1(z1 - lelo - bsh, P.nz);
+        const R wa = low ? omf : fr, wb = low ? fr : omf;
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int i = 0; i < 19; i++) {
-                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, P.nx - 1, y, za);
-                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, P.nx - 1, y, zb);
-                lo[c][i] = omf * a + fr * b;
+                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, za);
+                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, zb);
+                n[c][i] = wa * a + wb * b;
             }
+        if (low) {
+            R base[2][19];
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int i = 0; i < 19; i++) base[c][i] = n[c][i];
+            le_correct<R, Dev>(P, L, n, base, true);
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int i = 0; i < 19; i++) lo_done[c][i][j] = n[c][i];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    if (!low) {
         R base[2][19];
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
-            for (int i = 0; i < 19; i++) base[c][i] = lo[c][i];
-        le_correct<R, Dev>(P, L, lo, base, true);
+            for (int i = 0; i < 19; i++) base[c][i] = lo_done[c][i][j];
+        le_correct<R, Dev>(P, L, n, base, false);
     }
-    {
-        int za = wrapz1(z1 - lelo - 1 - bsh, P.nz), zb = wrapz1(z1 - lelo - bsh, P.nz);
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-#pragma unroll
-            for (int i = 0; i < 19; i++) {
-                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, 0, y, za);
-                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, 0, y, zb);
-                hi[c][i] = fr * a + omf * b;
-            }
-        le_correct<R, Dev>(P, L, hi, lo, false);
-    }
+    const int fc = low ? 0 : 1;
     if constexpr (!Post) {
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
-            for (int i = 0; i < 19; i++) {
-                face_pc[((uint32_t)(0 * 2 + c) * 19u + (uint32_t)i) * plane + t] = lo[c][i];
-                face_pc[((uint32_t)(1 * 2 + c) * 19u + (uint32_t)i) * plane + t] = hi[c][i];
-            }
+            for (int i = 0; i < 19; i++) face_pc[((uint32_t)(fc * 2 + c) * 19u + (uint32_t)i) * plane + t] = n[c][i];
     } else {
 #pragma unroll
-        for (int fc = 0; fc < 2; fc++) {
+        for (int c = 0; c < 2; c++) {
+            R rho = R(0);
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                R rho = R(0);
-#pragma unroll
-                for (int i = 0; i < 19; i++) rho = rho + (fc == 0 ? lo[c][i] : hi[c][i]) * R(GD[i]);
-                face_aux[(uint32_t)(fc * 4 + c) * plane + t] = psi_of<Dev>(rho, P.emr0[c]);
-                face_aux[(uint32_t)(fc * 4 + 2 + c) * plane + t] = rho;
-            }
+            for (int i = 0; i < 19; i++) rho = rho + n[c][i] * R(GD[i]);
+            face_aux[(uint32_t)(fc * 4 + c) * plane + t] = psi_of<Dev>(rho, P.emr0[c]);
+            face_aux[(uint32_t)(fc * 4 + 2 + c) * plane + t] = rho;
         }
     }
 }

@@ -330,12 +330,12 @@ class LauncherT : public Launcher {
     void le_faces(const DevBufs &b, bool post, hipStream_t s) override
     {
         unsigned plane = (unsigned)P.ny * (unsigned)P.nz;
-        unsigned nb = (plane + 127) / 128;
+        unsigned nb = (plane + LE_SITES - 1) / LE_SITES;
         const R *f = (const R *)b.f[b.cur];
         if (post)
-            k_le_faces<R, Dev, true><<<nb, 128, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+            k_le_faces<R, Dev, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
         else
-            k_le_faces<R, Dev, false><<<nb, 128, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+            k_le_faces<R, Dev, false><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
     }
     // LDS of k_fused2: the population stage plus the psi ring
     template <int TX, int TY>
