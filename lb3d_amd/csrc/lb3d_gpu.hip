@@ -99,7 +99,22 @@ struct lb3d_ctx {
     // field), so a captured two-step graph finds the live field where it
     // expects it whatever the host did before
     void *dbk[2] = {nullptr, nullptr};
+    // Lees-Edwards: the edge columns b.edge_pc hold f[cur]'s (written by the
+    // two-pass collide, else gathered before the next halo-plane pass)
+    bool edges_ok = false;
 };
+
+// The halo-plane kernels read the compact edge columns on the two-pass
+// plan, whose moments and collide passes write them.
+static bool compact_edges(const lb3d_ctx *c) { return !c->plan.fused; }
+
+static void ensure_edges(lb3d_ctx *c)
+{
+    if (c->le && compact_edges(c) && !c->edges_ok) {
+        c->L->le_gather_edges(c->b, c->stream);
+        c->edges_ok = true;
+    }
+}
 
 // Points b.db / b.db_alt at the dipole fields of the current parity.
 static void set_cur(lb3d_ctx *c, int cur)
@@ -306,6 +321,9 @@ int lb3d_gpu_create(const lb3d_params *p, lb3d_ctx **out)
         return cleanup(LB3D_ENOMEM, std::string("hipMalloc db: ") + hipGetErrorString(e));
     set_cur(c, 0);
     if (c->le) {
+        if ((e = hipMalloc(&c->b.edge_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
+            (e = hipMalloc(&c->b.edge_adv, 2 * 2 * 19 * plane * rs)) != hipSuccess)
+            return cleanup(LB3D_ENOMEM, std::string("hipMalloc LE edge columns: ") + hipGetErrorString(e));
         if ((e = hipMalloc(&c->b.face_pc, 2 * 2 * 19 * plane * rs)) != hipSuccess ||
             (e = hipMalloc(&c->b.face_aux, 2 * 4 * plane * rs)) != hipSuccess)
             return cleanup(LB3D_ENOMEM, std::string("hipMalloc LE planes: ") + hipGetErrorString(e));
@@ -341,6 +359,8 @@ int lb3d_gpu_destroy(lb3d_ctx *c)
     if (c->b.aux) (void)hipFree(c->b.aux);
     for (int k = 0; k < 2; k++)
         if (c->dbk[k]) (void)hipFree(c->dbk[k]);
+    if (c->b.edge_pc) (void)hipFree(c->b.edge_pc);
+    if (c->b.edge_adv) (void)hipFree(c->b.edge_adv);
     if (c->b.face_pc) (void)hipFree(c->b.face_pc);
     if (c->b.face_aux) (void)hipFree(c->b.face_aux);
     if (c->b.le) (void)hipFree(c->b.le);
@@ -418,6 +438,7 @@ int lb3d_gpu_upload_sites(lb3d_ctx *c, const double *N, int32_t h, int32_t sd, i
     c->state = IDLE;
     c->aux_valid = false;
     c->aux_from_prev = false;
+    c->edges_ok = false;
     return LB3D_OK;
 }
 
@@ -437,6 +458,7 @@ int lb3d_gpu_init_noise(lb3d_ctx *c, uint64_t seed, const double rho_mean[3], co
     c->state = IDLE;
     c->aux_valid = false;
     c->aux_from_prev = false;
+    c->edges_ok = false;
     return LB3D_OK;
 }
 
@@ -611,8 +633,9 @@ int lb3d_gpu_bc_apply(lb3d_ctx *c, int32_t stage, int32_t nt)
     if (stage == 6) {
         if (c->state != IDLE) return fail(LB3D_ESTATE, "bc_apply(6) must follow a collision or init");
         if (c->le) {
+            ensure_edges(c);
             launch_le_neighbours(c, nt);
-            c->L->le_faces(c->b, false, c->stream);
+            c->L->le_faces(c->b, false, compact_edges(c), c->stream);
             HIP_TRY(hipGetLastError());
         }
         c->nt = nt;
@@ -620,7 +643,7 @@ int lb3d_gpu_bc_apply(lb3d_ctx *c, int32_t stage, int32_t nt)
     } else if (stage == 8) {
         if (c->state != AFTER_ADV) return fail(LB3D_ESTATE, "bc_apply(8) must follow advection");
         if (c->le) {
-            c->L->le_faces(c->b, true, c->stream);
+            c->L->le_faces(c->b, true, compact_edges(c), c->stream);
             HIP_TRY(hipGetLastError());
         }
         c->state = AFTER_BC8;
@@ -677,6 +700,7 @@ int lb3d_gpu_collision(lb3d_ctx *c)
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
     flip(c);
+    c->edges_ok = compact_edges(c);
     c->state = IDLE;
     mark_stepped(c);
     return LB3D_OK;
@@ -685,15 +709,17 @@ int lb3d_gpu_collision(lb3d_ctx *c)
 // One whole timestep on the stream (graph-capturable: kernels only).
 static int enqueue_step(lb3d_ctx *c)
 {
+    // (Lees-Edwards on the two-pass plan: the edge columns are valid, see
+    // ensure_edges before the steps are enqueued)
     if (c->le) {
         launch_le_neighbours(c, -1);
-        c->L->le_faces(c->b, false, c->stream);
+        c->L->le_faces(c->b, false, compact_edges(c), c->stream);
     }
     if (c->plan.fused) {
-        if (c->le) c->L->le_faces(c->b, true, c->stream);
+        if (c->le) c->L->le_faces(c->b, true, false, c->stream);
     } else {
         c->L->moments(c->b, c->stream);
-        if (c->le) c->L->le_faces(c->b, true, c->stream);
+        if (c->le) c->L->le_faces(c->b, true, true, c->stream);
     }
     int rc = launch_collision(c);
     if (rc) return rc;
@@ -750,6 +776,7 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     int rc = sync_le_counter(c);
     if (rc) return rc;
     if ((rc = ensure_graph(c))) return rc;
+    ensure_edges(c);
     int left = nsteps;
     if (c->b.cur == 1) {
         if ((rc = enqueue_step(c))) return rc;
@@ -758,6 +785,7 @@ int lb3d_gpu_run_steps(lb3d_ctx *c, int32_t nsteps)
     for (; left >= 2; left -= 2) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
     if (left == 1 && (rc = enqueue_step(c))) return rc;
     HIP_TRY(hipGetLastError());
+    c->edges_ok = compact_edges(c);
     c->nt += nsteps;
     mark_stepped(c);
     return LB3D_OK;
@@ -848,9 +876,14 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         }
     } g{c, c->plan};
     // f[1-cur] and db_alt no longer hold the last step's input, so the
-    // moments of the last advection cannot be rebuilt from them afterwards
+    // moments of the last advection cannot be rebuilt from them afterwards;
+    // the edge columns are gathered for the trials and are stale after them
     c->aux_valid = false;
     c->aux_from_prev = false;
+    if (c->le) {
+        c->L->le_gather_edges(c->b, c->stream);
+        c->edges_ok = false;
+    }
     Events ev(2);
     HIP_TRY(ev.create());
     hipEvent_t e0 = ev[0], e1 = ev[1];
@@ -861,9 +894,12 @@ int lb3d_gpu_autotune(lb3d_ctx *c, int32_t trial_steps, char *report, int32_t re
         c->plan = q;
         c->L->xg = q.xg;
         auto trial = [&]() {
-            if (c->le) c->L->le_faces(c->b, false, c->stream);
+            // (the edge columns are those of the state, gathered before the
+            // trials; the two-pass trials' collide overwrites them with
+            // values of the trial, which are not the state's)
+            if (c->le) c->L->le_faces(c->b, false, !q.fused, c->stream);
             if (!q.fused) c->L->moments(c->b, c->stream);
-            if (c->le) c->L->le_faces(c->b, true, c->stream);
+            if (c->le) c->L->le_faces(c->b, true, !q.fused, c->stream);
             return launch_collision(c);
         };
         if (int rc = trial()) return rc;  // warm-up
@@ -939,6 +975,7 @@ int lb3d_gpu_set_plan(lb3d_ctx *c, int32_t fused, int32_t tx, int32_t ty, int32_
     }
     c->plan = q;
     c->L->xg = q.xg;
+    c->edges_ok = false;
     drop_graph(c);
     return LB3D_OK;
 }
@@ -1108,7 +1145,8 @@ int lb3d_gpu_le_halo(lb3d_ctx *c, double *out)
     int rc = sync_le_counter(c);
     if (rc) return rc;
     // the stage-6 x-halo planes of the current populations and offset
-    c->L->le_faces(c->b, false, c->stream);
+    ensure_edges(c);
+    c->L->le_faces(c->b, false, compact_edges(c), c->stream);
     HIP_TRY(hipGetLastError());
     const size_t plane = (size_t)c->p.ny * c->p.nz;
     const size_t n = 2 * 2 * 19 * plane;
@@ -1290,7 +1328,8 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
             launch_le_neighbours(c, -1);
             HIP_TRY(rec(st, K_LEN, false));
             HIP_TRY(rec(st, K_LE6, true));
-            c->L->le_faces(c->b, false, c->stream);
+            ensure_edges(c);
+            c->L->le_faces(c->b, false, compact_edges(c), c->stream);
             HIP_TRY(rec(st, K_LE6, false));
             mark(K_LEN);
             mark(K_LE6);
@@ -1303,7 +1342,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
         }
         if (c->le) {
             HIP_TRY(rec(st, K_LE8, true));
-            c->L->le_faces(c->b, true, c->stream);
+            c->L->le_faces(c->b, true, compact_edges(c), c->stream);
             HIP_TRY(rec(st, K_LE8, false));
             mark(K_LE8);
         }
@@ -1326,6 +1365,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
             mark(K_COL);
         }
         flip(c);
+        c->edges_ok = compact_edges(c);
     }
     HIP_TRY(hipEventRecord(t1, c->stream));
     HIP_TRY(hipGetLastError());

@@ -328,11 +328,44 @@ __device__ __forceinline__ void zou_he_planes(const Phys<R> &P, R (&n)[19], int 
     if (z == P.nz - 1) zou_he_site<R, Dev>(n, P.zh_flux[c], P.zh_type, false, P.rho0[c]);
 }
 
+// Lees-Edwards edge columns: the populations of the x = nx-1 (col 0) and
+// x = 0 (col 1) sites in a compact site-major layout
+// edge[(col*ny*nz + z*ny + y)*38 + c*19 + i].  A site's 38 values share two
+// or three cache lines, so the one lane of a row that writes them touches
+// those lines only, and the halo-plane kernels read them contiguously (in
+// the lattice layout each of them lies on its own cache line).
+template <typename T, typename R>
+__device__ __forceinline__ T *edge_site(T *edge, const Phys<R> &P, int col, uint32_t q)
+{
+    return edge + ((uint32_t)col * ((uint32_t)P.ny * (uint32_t)P.nz) + q) * 38u;
+}
+
+// Stores population (c, i) of an edge-column site into the compact layout
+// from the register that already holds it; e is null off the edge columns
+// (and when no edge buffer is given), so the other lanes store nothing.
+template <typename R>
+struct EdgeSink {
+    R *e;
+    __device__ __forceinline__ void operator()(int c, int i, R v) const
+    {
+        if (e) e[c * 19 + i] = v;
+    }
+};
+template <typename R>
+__device__ __forceinline__ EdgeSink<R> edge_sink(R *edge, const Phys<R> &P, int x, int y, int z)
+{
+    R *e = nullptr;
+    if (edge && (x == 0 || x == P.nx - 1))  // nx >= 2: a site is at most one column
+        e = edge_site(edge, P, x == 0 ? 1 : 0, (uint32_t)z * (uint32_t)P.ny + (uint32_t)y);
+    return EdgeSink<R>{e};
+}
+
 template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false, bool Flat = false>
 __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, int x, int y, int z, R (&rho)[C],
-                                             R (&da)[3], const uint8_t *__restrict__ rock = nullptr)
+                                             R (&da)[3], const uint8_t *__restrict__ rock = nullptr,
+                                             R *__restrict__ edge = nullptr)
 {
     static_assert(!Rock || (C == 2 && !LE), "walls: periodic binary lattices");
 #pragma unroll
@@ -358,6 +391,29 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
             da[0] = da0;
             da[1] = da1;
             da[2] = da2;
+            return;
+        }
+    }
+    if constexpr (LE && C == 2) {
+        if (edge) {
+            // Lees-Edwards binary, with the streamed edge-column populations
+            // kept for the post-advection halo planes: every pull is issued
+            // before the first (edge-lane) store, so the stores wait once
+            R n[C][19];
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+#pragma unroll
+                for (int c = 0; c < C; c++) n[c][i] = pull<R, C, LE>(f, face_pc, P, c, i, x, y, z);
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+#pragma unroll
+                for (int c = 0; c < C; c++) rho[c] = rho[c] + n[c][i] * R(GD[i]);
+            const EdgeSink<R> es = edge_sink(edge, P, x, y, z);
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int i = 0; i < 19; i++) es(c, i, n[c][i]);
+            da[0] = da[1] = da[2] = R(0);
             return;
         }
     }
@@ -389,12 +445,13 @@ template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = fal
 __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, R *__restrict__ aux,
-                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr)
+                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr,
+                                             R *__restrict__ edge_adv = nullptr)
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
     R rho[C], da[3];
-    moments_core<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, x, y, z, rho, da, rock);
+    moments_core<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, x, y, z, rho, da, rock, edge_adv);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
@@ -410,11 +467,26 @@ template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = fal
 __global__ __launch_bounds__(256) void k_moments(Phys<R> P, const R *__restrict__ f,
                                                  const R *__restrict__ face_pc,
                                                  const R *__restrict__ db, R *__restrict__ aux,
-                                                 uint32_t xg, const uint8_t *__restrict__ rock)
+                                                 uint32_t xg, const uint8_t *__restrict__ rock,
+                                                 R *__restrict__ edge_adv = nullptr)
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
-    moments_site<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, aux, s, rock);
+    moments_site<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, aux, s, rock, edge_adv);
+}
+
+// Fills the edge columns of the pre-advection halo planes from the
+// populations f (when no collide of the two-pass plan has written them).
+template <typename R>
+__global__ __launch_bounds__(256) void k_le_gather_edges(Phys<R> P, const R *__restrict__ f, R *__restrict__ edge_pc)
+{
+    const uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // (col*plane + q)*38 + c*19 + i
+    if (t >= 2u * 38u * plane) return;
+    const uint32_t ci = t % 38u, q = (t / 38u) % plane, col = t / (38u * plane);
+    const int y = (int)(q % (uint32_t)P.ny), z = (int)(q / (uint32_t)P.ny);
+    const int x = col == 0 ? P.nx - 1 : 0;
+    edge_pc[t] = f[ci * P.ns + sidx(x, y, z, P.nx, P.ny)];
 }
 
 // --------------------------------------------------------------------------
@@ -766,9 +838,10 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
                                           R (&F)[C][3], int x, int y, int z, uint32_t s, R *__restrict__ fout,
                                           bool store = true, Sink sink = Sink());
 
-template <typename R, bool Dev, int C, int BD, bool Refetch = false, bool NT = false, typename Get>
+template <typename R, bool Dev, int C, int BD, bool Refetch = false, bool NT = false, typename Get,
+          typename Sink = NoSink>
 __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][3], int x, int y, int z,
-                                           uint32_t s, R *__restrict__ fout)
+                                           uint32_t s, R *__restrict__ fout, Sink sink = Sink())
 {
     R rho[C], mom[C][3];
 #pragma unroll
@@ -783,7 +856,7 @@ __device__ __forceinline__ void bgk_site_g(const Phys<R> &P, Get get, R (&F)[C][
         for (int c = 0; c < C; c++) acc_moments<R>(i, get(c, i), rho[c], mom[c]);
     }
     if constexpr (Refetch) asm volatile("" ::: "memory");
-    bgk_relax<R, Dev, C, BD, NT>(P, get, rho, mom, F, x, y, z, s, fout);
+    bgk_relax<R, Dev, C, BD, NT>(P, get, rho, mom, F, x, y, z, s, fout, true, sink);
 }
 
 // The relaxation of bgk_site_g from the moments (rho, mom) of the streamed
@@ -897,12 +970,13 @@ __device__ __forceinline__ void bgk_relax(const Phys<R> &P, Get get, const R (&r
     }
 }
 
-template <typename R, bool Dev, int C, int BD, bool NT = false>
+template <typename R, bool Dev, int C, int BD, bool NT = false, typename Sink = NoSink>
 __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], R (&F)[C][3],
-                                         int x, int y, int z, uint32_t s, R *__restrict__ fout)
+                                         int x, int y, int z, uint32_t s, R *__restrict__ fout,
+                                         Sink sink = Sink())
 {
     bgk_site_g<R, Dev, C, BD, false, NT>(P, [&](int c, int i) __attribute__((always_inline)) { return n[c][i]; }, F, x, y,
-                              z, s, fout);
+                              z, s, fout, sink);
 }
 
 // --------------------------------------------------------------------------
@@ -915,7 +989,8 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
                                              const R *__restrict__ aux,
                                              const R *__restrict__ face_aux,
                                              const R *__restrict__ db, R *__restrict__ fout,
-                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr)
+                                             uint32_t s, const uint8_t *__restrict__ rock = nullptr,
+                                             R *__restrict__ edge_pc = nullptr)
 {
     int x, y, z;
     site_xyz(s, P.nx, P.ny, x, y, z);
@@ -969,7 +1044,10 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
             return;
         }
     }
-    bgk_site<R, Dev, C, BD, (NT & 2) != 0>(P, n, F, x, y, z, s, fout);
+    if constexpr (LE && C == 2)  // the edge columns' post-collision populations
+        bgk_site<R, Dev, C, BD, (NT & 2) != 0>(P, n, F, x, y, z, s, fout, edge_sink(edge_pc, P, x, y, z));
+    else
+        bgk_site<R, Dev, C, BD, (NT & 2) != 0>(P, n, F, x, y, z, s, fout);
 }
 
 // NT: the populations' loads and stores with the non-temporal hint, for
@@ -980,14 +1058,15 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
                                                  const R *__restrict__ aux,
                                                  const R *__restrict__ face_aux,
                                                  const R *__restrict__ db, R *__restrict__ fout,
-                                                 uint32_t xg, const uint8_t *__restrict__ rock)
+                                                 uint32_t xg, const uint8_t *__restrict__ rock,
+                                                 R *__restrict__ edge_pc = nullptr)
 {
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     // measured on MI355X (FP32 and FP64): ternary loads and stores (128^3:
     // -6 to -8% per step), binary stores only (256^3 FP64: -2%)
     constexpr int nt = !NT ? 0 : (C == 3 ? 3 : 2);
-    collide_site<R, Dev, C, LE, BD, Rock, ZH, nt>(P, f, face_pc, aux, face_aux, db, fout, s, rock);
+    collide_site<R, Dev, C, LE, BD, Rock, ZH, nt>(P, f, face_pc, aux, face_aux, db, fout, s, rock, edge_pc);
 }
 
 // --------------------------------------------------------------------------
@@ -1541,10 +1620,13 @@ __device__ __forceinline__ int wrapz1(int k, int n)
 // x = nx+1 plane's, both interpolating at once; the high side's clamp then
 // reads the finished low site from LDS.
 constexpr int LE_SITES = 64;
-template <typename R, bool Dev, bool Post>
+// Compact: the sources come from the edge columns (edge: the pre-advection
+// ones, or for Post the streamed ones) instead of the lattice.
+template <typename R, bool Dev, bool Post, bool Compact = false>
 __global__ __launch_bounds__(2 * LE_SITES) void k_le_faces(Phys<R> P, const LEState *__restrict__ Lp,
                                                            const R *__restrict__ f, R *__restrict__ face_pc,
-                                                           R *__restrict__ face_aux)
+                                                           R *__restrict__ face_aux,
+                                                           const R *__restrict__ edge = nullptr)
 {
     __shared__ R lo_done[2][19][LE_SITES];  // the corrected low-plane sites
     const uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
@@ -1565,12 +1647,20 @@ __global__ __launch_bounds__(2 * LE_SITES) void k_le_faces(Phys<R> P, const LESt
         const int za = low ? wrapz1(z1 + lelo + bsh, P.nz) : wrapz1(z1 - lelo - 1 - bsh, P.nz);
         const int zb = low ? wrapz1(z1 + lelo + 1 + bsh, P.nz) : wrapz1(z1 - lelo - bsh, P.nz);
         const R wa = low ? omf : fr, wb = low ? fr : omf;
+        const int col = low ? 0 : 1;
+        const uint32_t qa = (uint32_t)za * (uint32_t)P.ny + (uint32_t)y, qb = (uint32_t)zb * (uint32_t)P.ny + (uint32_t)y;
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int i = 0; i < 19; i++) {
-                R a = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, za);
-                R b = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, zb);
+                R a, b;
+                if constexpr (Compact) {
+                    a = edge_site(edge, P, col, qa)[c * 19 + i];
+                    b = edge_site(edge, P, col, qb)[c * 19 + i];
+                } else {
+                    a = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, za);
+                    b = le_src<R, Dev, Post>(P, f, face_pc, c, i, xs, y, zb);
+                }
                 n[c][i] = wa * a + wb * b;
             }
         if (low) {

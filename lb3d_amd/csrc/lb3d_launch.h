@@ -268,7 +268,7 @@ class LauncherT : public Launcher {
                                                                   nullptr);
         else if (le)
             k_moments<R, Dev, 2, true><<<tiles(), BLOCK, 0, s>>>(P, f, (const R *)b.face_pc, nullptr, (R *)b.aux, xg,
-                                                                 nullptr);
+                                                                 nullptr, (R *)b.edge_adv);
         else if (b.rock)
             k_moments<R, Dev, 2, false, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, nullptr, (R *)b.aux, xg,
                                                                         b.rock);
@@ -306,7 +306,8 @@ class LauncherT : public Launcher {
                 P, f, nullptr, (const R *)b.aux, nullptr, (const R *)b.db_alt, fo, xg, nullptr);
         else if (le)
             k_collide<R, Dev, 2, true, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
-                P, f, (const R *)b.face_pc, (const R *)b.aux, (const R *)b.face_aux, nullptr, fo, xg, nullptr);
+                P, f, (const R *)b.face_pc, (const R *)b.aux, (const R *)b.face_aux, nullptr, fo, xg, nullptr,
+                (R *)b.edge_pc);
         else if (b.rock)
             k_collide<R, Dev, 2, false, BD, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
                                                                             nullptr, nullptr, fo, xg, b.rock);
@@ -327,15 +328,25 @@ class LauncherT : public Launcher {
         else
             n ? collide_bd<BD_GENERAL, true>(b, s) : collide_bd<BD_GENERAL, false>(b, s);
     }
-    void le_faces(const DevBufs &b, bool post, hipStream_t s) override
+    void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s) override
     {
         unsigned plane = (unsigned)P.ny * (unsigned)P.nz;
         unsigned nb = (plane + LE_SITES - 1) / LE_SITES;
         const R *f = (const R *)b.f[b.cur];
-        if (post)
-            k_le_faces<R, Dev, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+        R *fp = (R *)b.face_pc, *fa = (R *)b.face_aux;
+        if (post && compact)
+            k_le_faces<R, Dev, true, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_adv);
+        else if (post)
+            k_le_faces<R, Dev, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa);
+        else if (compact)
+            k_le_faces<R, Dev, false, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_pc);
         else
-            k_le_faces<R, Dev, false><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, (R *)b.face_pc, (R *)b.face_aux);
+            k_le_faces<R, Dev, false><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa);
+    }
+    void le_gather_edges(const DevBufs &b, hipStream_t s) override
+    {
+        unsigned n = 2u * 2u * 19u * (unsigned)P.ny * (unsigned)P.nz;
+        k_le_gather_edges<R><<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(P, (const R *)b.f[b.cur], (R *)b.edge_pc);
     }
     // LDS of k_fused2: the population stage plus the psi ring
     template <int TX, int TY>

@@ -22,6 +22,9 @@ struct DevBufs {
     // batch relaxes one field in place (db_alt == db)
     void *aux = nullptr, *db = nullptr, *face_pc = nullptr, *face_aux = nullptr;
     void *db_alt = nullptr;
+    // Lees-Edwards edge columns (x = nx-1 and x = 0) in a compact layout: the
+    // post-collision populations (edge_pc) and the streamed ones (edge_adv)
+    void *edge_pc = nullptr, *edge_adv = nullptr;
     const uint8_t *rock = nullptr;  // 1 at rock sites, or nullptr for a lattice without walls
     LEState *le = nullptr;
     int cur = 0;
@@ -38,7 +41,12 @@ class Launcher {
     virtual void moments(const DevBufs &b, hipStream_t s) = 0;
     virtual void dipole(const DevBufs &b, hipStream_t s) = 0;
     virtual void collide(const DevBufs &b, hipStream_t s) = 0;  // f[cur] -> f[1-cur]
-    virtual void le_faces(const DevBufs &b, bool post, hipStream_t s) = 0;
+    // Lees-Edwards x-halo planes, pre- (post = false) or post-advection;
+    // compact: from the edge columns (b.edge_pc written by the two-pass
+    // collide or le_gather_edges, b.edge_adv by the moments pass)
+    virtual void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s) = 0;
+    // b.edge_pc from the populations f[cur]
+    virtual void le_gather_edges(const DevBufs &b, hipStream_t s) = 0;
     // the Shan-Chen force field from b.aux and b.db into out[(c*3 + k)*ns + s]
     // (working type R), periodic in x
     virtual void forces(const DevBufs &b, void *out, hipStream_t s) = 0;
