@@ -187,8 +187,9 @@ def config_accuracy(name, dims, steps, kw, amp, checkpoints):
 
 
 def traffic_probe(n, plan):
-    """Child run under rocprofv3 --pmc: a few steps of the FP32 lattice with the
-    given plan, then a device copy of known size (the counter calibration)."""
+    """Child run under rocprofv3 --pmc: a few steps (plan["steps"], default 4)
+    of the FP32 lattice with the given plan, then a device copy of known size
+    (the counter calibration)."""
     import torch
     from lb3d_amd.gpu import LB3D_FP32_DEV, Lattice
 
@@ -196,7 +197,7 @@ def traffic_probe(n, plan):
     g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
     g.bc_init()
     g.set_plan(plan["mode"], plan["tx"], plan["ty"], max(plan["lz"], 1), xcd_group=plan["xg"])
-    g.time_loop(4)
+    g.time_loop(plan.get("steps", 4))
     g.synchronize()
     g.close()
     a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
@@ -226,24 +227,86 @@ def ternary_probe(n):
     g.close()
 
 
-def le_probe(precision_name):
-    """Child run for rocprofv3 passes: the Lees-Edwards 256x64x64 binary lattice
-    (shear velocity 0.05, g_br 0.08) in the given form on the tuner's plan,
-    5 steps, then a device copy of known size (the counter calibration)."""
-    import torch  # noqa: F401  (first: see copy_calibration)
-    from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, LB3D_FP32_DEV, LB3D_FP64, Lattice
+FP64_PROBE_PLANS = ((0, 64, 4, 1, 64), (3, 256, 2, 128, 64), (2, 256, 2, 128, 64), (1, 32, 8, 64, 32))
 
-    prec = LB3D_FP32_DEV if precision_name == "fp32" else LB3D_FP64
-    g = Lattice(256, 64, 64, ncomp=2, precision=prec, g_br=0.08, bdist=2, boundary=LB3D_BC_LEES_EDWARDS,
-                shear_u=0.05)
+
+def fp64_probe():
+    """Child run for rocprofv3 passes: the binary 256^3 lattice in FP64, 3
+    steps on each of the plans FP64_PROBE_PLANS (set_plan arguments: the
+    two-pass plan, then the one-pass kernel with the streamed populations
+    pulled again, in registers, in an LDS stage)."""
+    import torch  # noqa: F401  (first: see copy_calibration)
+    from lb3d_amd.gpu import LB3D_FP64, Lattice
+
+    g = Lattice(256, 256, 256, ncomp=2, precision=LB3D_FP64, g_br=0.08, bdist=2)
     g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
     g.bc_init()
-    print(g.autotune(trial_steps=3).strip().splitlines()[-1], flush=True)
-    g.time_loop(5)
+    for plan in FP64_PROBE_PLANS:
+        g.set_plan(*plan)
+        g.time_loop(3)
+        g.synchronize()
+    g.close()
+    copy_calibration()
+
+
+LE_DIMS = (256, 64, 64)
+LE_KW = dict(ncomp=2, g_br=0.08, bdist=2, shear_u=0.05)
+
+
+def le_lattice(precision):
+    from lb3d_amd.gpu import LB3D_BC_LEES_EDWARDS, Lattice
+
+    g = Lattice(*LE_DIMS, precision=precision, boundary=LB3D_BC_LEES_EDWARDS, **LE_KW)
+    g.init_noise(1, rho_mean=(0.7, 0.7, 0.7), amp=(0.01, 0.01, 0.0), dipoles=False)
+    g.bc_init()
+    return g
+
+
+def le_probe(precision_name):
+    """Child run for rocprofv3 passes: the Lees-Edwards 256x64x64 binary lattice
+    (shear velocity 0.05, g_br 0.08) in the given form on the two-pass plan
+    (which the tuner picks on this lattice), 10 steps, 5 profiled steps, then
+    a device copy of known size (the counter calibration)."""
+    import torch  # noqa: F401  (first: see copy_calibration)
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64
+
+    g = le_lattice(LB3D_FP32_DEV if precision_name == "fp32" else LB3D_FP64)
+    g.set_plan(0)
+    g.time_loop(10)
     g.synchronize()
     print(g.profile_steps(5), flush=True)
     g.close()
     copy_calibration()
+
+
+def run_le(steps):
+    """The Lees-Edwards lattice in both forms on the tuner's plan: `steps`
+    timed steps (HIP events), the per-kernel times, the step's fraction of
+    the HBM peak at the form's algorithmic bytes per site."""
+    from lb3d_amd.gpu import LB3D_FP32_DEV, LB3D_FP64
+
+    out = {}
+    ns = int(np.prod(LE_DIMS))
+    for tag, prec, rb in (("fp32", LB3D_FP32_DEV, 4), ("fp64", LB3D_FP64, 8)):
+        g = le_lattice(prec)
+        rep = g.autotune(trial_steps=3).strip().splitlines()
+        chosen = rep[-1]
+        one_pass = [ln for ln in rep[:-1] if ln.startswith("fused")]
+        g.time_loop(10)
+        g.synchronize()
+        ms = g.time_loop_timed(steps) / steps
+        kern, _ = g.profile_steps(5)
+        g.close()
+        B = bytes_per_site(2, False, rb)
+        out[tag] = dict(value=round(ns / (ms * 1e-3) / 1e6, 1), unit="MLUPS", ms_per_step=round(ms, 5),
+                        bytes_per_site=B, step_frac=round(B * ns / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                        plan=chosen, kernels_ms={k: round(v, 5) for k, v in kern.items()},
+                        best_one_pass_trial=min(one_pass, key=lambda ln: float(ln.split()[-2])) if one_pass else None)
+    out["workload"] = ("binary Shan-Chen D3Q19 256x64x64, Lees-Edwards planes at x (shear velocity 0.05), "
+                       "g_br=0.08, tau=1, bdist=2, rho=0.7+-0.01 (seed 1)")
+    out["steps"] = steps
+    out["timing"] = "HIP events on the library stream around the timed steps / steps; kernels over 5 steps"
+    return out
 
 
 ENS_COMMON = dict(ncomp=3, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2)
@@ -645,6 +708,7 @@ def main():
     ap.add_argument("--size", type=int, default=256, help="edge of the binary spinodal lattice")
     ap.add_argument("--binary-steps", type=int, default=200)
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--le-steps", type=int, default=500, help="timed steps of the Lees-Edwards lattice")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="FP32 against FP64 after this many steps, binary lattice and ensemble (0: skip)")
     ap.add_argument("--no-configs", action="store_true",
@@ -661,6 +725,7 @@ def main():
     ap.add_argument("--ternary-probe", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--ensemble-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--le-probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--fp64-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (a kernel of known "
                          "traffic for scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -686,6 +751,9 @@ def main():
         return
     if args.le_probe is not None:
         le_probe(args.le_probe)
+        return
+    if args.fp64_probe:
+        fp64_probe()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -776,6 +844,11 @@ def main():
     print(f"[bench] binary {n}^3: FP32 {binary['value']} MLUPS, FP64 {binary['fp64_plain']['value']} MLUPS",
           file=sys.stderr, flush=True)
 
+    print("[bench] Lees-Edwards 256x64x64: FP32 deviation and FP64 plain", file=sys.stderr, flush=True)
+    le = run_le(args.le_steps)
+    print(f"[bench] Lees-Edwards: FP32 {le['fp32']['value']} MLUPS ({le['fp32']['step_frac']}), "
+          f"FP64 {le['fp64']['value']} MLUPS ({le['fp64']['step_frac']})", file=sys.stderr, flush=True)
+
     # ---- FP32 against FP64 per configuration ------------------------------
     accuracy = {}
     if args.accuracy_steps > 0:
@@ -831,6 +904,7 @@ def main():
         "phases_note": ("HIP events around each launch over 3 steps launched one by one after the timed loop "
                         "(not graph-replayed); roofline.mean_launch_ms is the timed loop's per-step time"),
         "binary_256": binary,
+        "lees_edwards_256x64x64": le,
         "accuracy_fp32_vs_fp64": accuracy,
     }
     print(json.dumps(line), flush=True)

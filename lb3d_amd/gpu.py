@@ -313,7 +313,7 @@ class Lattice:
     def autotune(self, trial_steps=3):
         """Time every execution plan on this lattice and keep the fastest;
         returns the tuner's report."""
-        buf = ctypes.create_string_buffer(8192)
+        buf = ctypes.create_string_buffer(32768)
         check(lib().lb3d_gpu_autotune(self._h, int(trial_steps), buf, len(buf)), "autotune")
         return buf.value.decode()
 
@@ -491,7 +491,7 @@ class Batch:
         check(lib().lb3d_gpu_batch_set_groups(self._h, int(group_bytes)), "batch_set_groups")
 
     def autotune(self, trial_steps=3):
-        buf = ctypes.create_string_buffer(8192)
+        buf = ctypes.create_string_buffer(32768)
         check(lib().lb3d_gpu_batch_autotune(self._h, int(trial_steps), buf, len(buf)), "batch_autotune")
         return buf.value.decode()
 
