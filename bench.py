@@ -651,17 +651,29 @@ def host_threads():
     return len(os.sched_getaffinity(0))
 
 
+def cpu_model():
+    cpu = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    return next((ln.split(":", 1)[1].strip() for ln in cpu.splitlines() if ln.startswith("model name")), "")
+
+
 def cpu_baseline(sample_lattices, sample_steps, single_n, single_steps):
-    """The oracle (FP64 C restatement of the reference loop) on a bounded
-    sample of the ensemble: its first `sample_lattices` lattices with edges up
-    to 56, `sample_steps` steps each, on all host threads; then one lattice of
-    edge single_n on one core."""
+    """The CPU path on a bounded sample of the ensemble (its first
+    `sample_lattices` lattices with edges up to 56, `sample_steps` steps
+    each, on all host threads; then the lattice of edge single_n on one core):
+    the reference LB3D itself (built by oracle/ref_build.sh, `mpirun -n
+    <threads>`, the step-loop time of its own timer) when it is built here,
+    and the oracle (the FP64 C restatement, OpenMP over sites) beside it."""
+    from oracle import refrun
     from oracle.pyoracle import Oracle, set_threads
 
     edges, g_br = ensemble_spec(48)
     kw = dict(ENS_COMMON)
+    th = host_threads()
+    picks = [(k, e) for k, e in enumerate(edges) if e <= 56][:sample_lattices]
+    k1 = next(k for k, e in enumerate(edges) if e == single_n) if single_n in edges else 0
+    names = ", ".join(str(e) for _, e in picks)
 
-    def timed(picks, steps, threads):
+    def port(picks, steps, threads):
         set_threads(threads)
         n_upd, t = 0, 0.0
         for k, e in picks:
@@ -675,22 +687,47 @@ def cpu_baseline(sample_lattices, sample_steps, single_n, single_steps):
             o.close()
         return n_upd / t / 1e6, t
 
-    th = host_threads()
-    picks = [(k, e) for k, e in enumerate(edges) if e <= 56][:sample_lattices]
-    v, dt = timed(picks, sample_steps, th)
-    k1 = next(k for k, e in enumerate(edges) if e == single_n) if single_n in edges else 0
-    v1, dt1 = timed([(k1, single_n)], single_steps, 1)
-    cpu = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
-    model = next((ln.split(":", 1)[1].strip() for ln in cpu.splitlines() if ln.startswith("model name")), "")
-    return dict(value=round(v, 4), unit="MLUPS", cores=th, kind="port",
-                sample=f"oracle (FP64 C restatement of the reference loop, OpenMP over sites, {th} threads): "
-                       f"the ensemble's first {len(picks)} lattices with edges <= 56 "
-                       f"({', '.join(str(e) for _, e in picks)}), {sample_steps} steps each, same physics and "
-                       "initial state as the workload",
-                seconds=round(dt, 3), cpu_model=model,
-                single_core=dict(value=round(v1, 4), unit="MLUPS", cores=1,
-                                 sample=f"ternary {single_n}^3 lattice of the ensemble, {single_steps} steps",
-                                 seconds=round(dt1, 3)))
+    def reference(picks, steps, ranks):
+        n_upd, t = 0, 0.0
+        rkw = {k: v for k, v in kw.items() if k in ("g_bs", "g_ss", "tau_d", "beta", "d_0", "bdist")}
+        for k, e in picks:
+            secs, _ = refrun.run(f"ens{k}", (e, e, e), steps, True, ranks=ranks, g_br=g_br[k], **rkw)
+            t += secs
+            n_upd += e ** 3 * steps
+        return n_upd / t / 1e6, t
+
+    v, dt = port(picks, sample_steps, th)
+    v1, dt1 = port([(k1, single_n)], single_steps, 1)
+    port_line = dict(value=round(v, 4), unit="MLUPS", cores=th, kind="port",
+                     sample=f"oracle (FP64 C restatement of the reference loop, OpenMP over sites, {th} threads): "
+                            f"the ensemble's first {len(picks)} lattices with edges <= 56 ({names}), "
+                            f"{sample_steps} steps each, same physics and initial state as the workload",
+                     seconds=round(dt, 3),
+                     single_core=dict(value=round(v1, 4), unit="MLUPS", cores=1,
+                                      sample=f"ternary {single_n}^3 lattice of the ensemble, {single_steps} steps",
+                                      seconds=round(dt1, 3)))
+    ok, why = refrun.available(True)
+    if ok:
+        try:
+            rv, rdt = reference(picks, sample_steps, th)
+            rv1, rdt1 = reference([(k1, single_n)], single_steps, 1)
+        except Exception as e:  # the baseline is reported, never the product: say why it is missing
+            ok, why = False, f"reference run failed: {e}"[:400]
+    if not ok:
+        port_line.update(cpu_model=cpu_model(), reference_unavailable=why)
+        return port_line
+    return dict(value=round(rv, 4), unit="MLUPS", cores=th, kind="reference",
+                sample=f"the reference LB3D (Fortran + MPI, built from its sources with amdflang and MPICH), "
+                       f"mpirun -n {th}, ternary build: the ensemble's first {len(picks)} lattices with edges "
+                       f"<= 56 ({names}), {sample_steps} steps each, the same physics from the reference's "
+                       "INIT_RAND state (mean density 0.7); time = its own step-loop timer "
+                       "(code/lb3d_time_loop.F90:56-95)",
+                seconds=round(rdt, 3), cpu_model=cpu_model(),
+                single_core=dict(value=round(rv1, 4), unit="MLUPS", cores=1,
+                                 sample=f"ternary {single_n}^3 lattice of the ensemble, {single_steps} steps, "
+                                        "mpirun -n 1",
+                                 seconds=round(rdt1, 3)),
+                port=port_line)
 
 
 PHASE_KERNELS = {"persistent": dict(moments="k_batch_moments", dipole="k_batch_dipole", collide="k_batch_collide"),
@@ -715,7 +752,7 @@ def main():
                     help="skip the FP32-vs-FP64 accuracy runs of the ternary and Lees-Edwards configurations")
     ap.add_argument("--config-steps-scale", type=float, default=1.0)
     ap.add_argument("--cpu-sample-lattices", type=int, default=8)
-    ap.add_argument("--cpu-sample-steps", type=int, default=10)
+    ap.add_argument("--cpu-sample-steps", type=int, default=30)
     ap.add_argument("--cpu-single-size", type=int, default=32)
     ap.add_argument("--cpu-single-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
