@@ -81,6 +81,30 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint32_t stream, u
     return (double)(h >> 11) * 0x1.0p-53;
 }
 
+// Wavefront reductions for the diagnostics: a butterfly over the 64 lanes
+// (cross-lane permutes, no LDS), after which every lane holds the result;
+// the block then combines its waves' results in wave order.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    return v;
+}
+// the largest m over the wave; on equal m the larger key (the site that
+// comes later in the reference's loop), the same pair on every lane
+__device__ __forceinline__ void wave_argmax(double &m, double &key)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double m1 = __shfl_xor(m, o, 64), k1 = __shfl_xor(key, o, 64);
+        if (m1 > m || (m1 == m && k1 > key)) {
+            m = m1;
+            key = k1;
+        }
+    }
+}
+constexpr int RED_WAVES = 256 / 64;  // waves of a 256-thread diagnostics block
+
 template <typename R, bool Dev, int C>
 __global__ __launch_bounds__(256) void k_init_noise(Phys<R> P, uint64_t seed, double m0, double m1,
                                                     double m2, double a0, double a1, double a2,
@@ -122,7 +146,7 @@ __global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restr
                                                      const uint8_t *__restrict__ rock,
                                                      double *__restrict__ partials)
 {
-    __shared__ double red[NDIAG][256];
+    __shared__ double red[NDIAG][RED_WAVES];
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NDIAG];
     for (int k = 0; k < NDIAG; k++) v[k] = 0.0;
@@ -155,15 +179,18 @@ __global__ __launch_bounds__(256) void k_diagnostics(Phys<R> P, const R *__restr
         v[7] = isneg ? 1.0 : 0.0;
         v[8] = fast ? 1.0 : 0.0;
     }
-    for (int k = 0; k < NDIAG; k++) red[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int k = 0; k < NDIAG; k++) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + w];
-        __syncthreads();
+    const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;
+    for (int k = 0; k < NDIAG; k++) {
+        const double t = wave_sum(v[k]);
+        if (ln == 0) red[k][wv] = t;
     }
+    __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 0; k < NDIAG; k++) partials[(size_t)blockIdx.x * NDIAG + k] = red[k][0];
+        for (int k = 0; k < NDIAG; k++) {
+            double t = red[k][0];
+            for (int w = 1; w < RED_WAVES; w++) t = t + red[k][w];
+            partials[(size_t)blockIdx.x * NDIAG + k] = t;
+        }
 }
 
 // Full sanity check (code/lb3d_sanity.F90:131-224) on the current
@@ -179,7 +206,7 @@ __global__ __launch_bounds__(256) void k_sanity(Phys<R> P, const R *__restrict__
                                                 const uint8_t *__restrict__ rock,
                                                 double *__restrict__ partials)
 {
-    __shared__ double red[NSAN][256];
+    __shared__ double red[NSAN][RED_WAVES];
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     double v[NSAN];
     for (int k = 0; k < NSAN; k++) v[k] = k < SAN_MAX ? 0.0 : -1.0;
@@ -223,25 +250,39 @@ __global__ __launch_bounds__(256) void k_sanity(Phys<R> P, const R *__restrict__
         v[7] = isneg ? 1.0 : 0.0;
         v[8] = fast ? 1.0 : 0.0;
     }
-    for (int k = 0; k < NSAN; k++) red[k][threadIdx.x] = v[k];
+    const unsigned wv = threadIdx.x / 64u, ln = threadIdx.x % 64u;
+    for (int k = 0; k < SAN_MAX; k++) {
+        const double t = wave_sum(v[k]);
+        if (ln == 0) red[k][wv] = t;
+    }
+    for (int k = 0; k < 9; k++) {
+        double m = v[SAN_MAX + k], key = v[SAN_KEY + k];
+        wave_argmax(m, key);
+        if (ln == 0) {
+            red[SAN_MAX + k][wv] = m;
+            red[SAN_KEY + k][wv] = key;
+        }
+    }
     __syncthreads();
-    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            const unsigned o = threadIdx.x + w;
-            for (int k = 0; k < SAN_MAX; k++) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][o];
-            for (int k = 0; k < 9; k++) {
-                double m0 = red[SAN_MAX + k][threadIdx.x], m1 = red[SAN_MAX + k][o];
-                double k0 = red[SAN_KEY + k][threadIdx.x], k1 = red[SAN_KEY + k][o];
-                if (m1 > m0 || (m1 == m0 && k1 > k0)) {
-                    red[SAN_MAX + k][threadIdx.x] = m1;
-                    red[SAN_KEY + k][threadIdx.x] = k1;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < SAN_MAX; k++) {
+            double t = red[k][0];
+            for (int w = 1; w < RED_WAVES; w++) t = t + red[k][w];
+            partials[(size_t)blockIdx.x * NSAN + k] = t;
+        }
+        for (int k = 0; k < 9; k++) {
+            double m = red[SAN_MAX + k][0], key = red[SAN_KEY + k][0];
+            for (int w = 1; w < RED_WAVES; w++) {
+                const double m1 = red[SAN_MAX + k][w], k1 = red[SAN_KEY + k][w];
+                if (m1 > m || (m1 == m && k1 > key)) {
+                    m = m1;
+                    key = k1;
                 }
             }
+            partials[(size_t)blockIdx.x * NSAN + SAN_MAX + k] = m;
+            partials[(size_t)blockIdx.x * NSAN + SAN_KEY + k] = key;
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < NSAN; k++) partials[(size_t)blockIdx.x * NSAN + k] = red[k][0];
 }
 
 template <typename R, bool Dev>
