@@ -533,8 +533,19 @@ def hbm_ceiling(nbytes=2 << 30, reps=10):
     rc = lib.lb3d_bw_probe(nbytes, reps, out)
     if rc:
         raise RuntimeError(f"lb3d_bw_probe failed with HIP error {rc}")
-    return dict(copy=round(out[0], 1), read=round(out[1], 1), write=round(out[2], 1), unit="GB/s",
-                probe=f"float4 grid-stride sweeps over {nbytes >> 20} MiB buffers, best of 4 grid sizes")
+    res = dict(copy=round(out[0], 1), read=round(out[1], 1), write=round(out[2], 1), unit="GB/s",
+               probe=f"float4 grid-stride sweeps over {nbytes >> 20} MiB buffers, best of 4 grid sizes")
+    lib.lb3d_bw_probe_planes.restype = ctypes.c_int
+    lib.lb3d_bw_probe_planes.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    for planes in (38, 57):
+        v = ctypes.c_double()
+        rc = lib.lb3d_bw_probe_planes(nbytes, planes, reps, ctypes.byref(v))
+        if rc:
+            raise RuntimeError(f"lb3d_bw_probe_planes failed with HIP error {rc}")
+        res[f"plane_copy_{planes}"] = round(v.value, 1)
+    res["plane_copy_probe"] = ("the population layout: one thread per site copies its 38 (binary FP32) or 57 "
+                               "(ternary FP32) values between planes ns apart, read + write bytes")
+    return res
 
 
 def ensemble_spec(nlat=48, seed=1):
@@ -839,7 +850,9 @@ def main():
     ceiling = hbm_ceiling()
     ceiling["step_algorithmic_frac_of_copy"] = round(achieved / ceiling["copy"], 4)
     if roofline["traffic"]:
-        ceiling["step_traffic_frac_of_copy"] = round(roofline["traffic"] / (ev * 1e-3) / 1e9 / ceiling["copy"], 4)
+        rate = roofline["traffic"] / (ev * 1e-3) / 1e9
+        ceiling["step_traffic_frac_of_copy"] = round(rate / ceiling["copy"], 4)
+        ceiling["step_traffic_frac_of_plane_copy_57"] = round(rate / ceiling["plane_copy_57"], 4)
 
     # ---- the binary spinodal lattice: FP32 deviation and FP64 plain ------
     n, ncomp = args.size, 2
