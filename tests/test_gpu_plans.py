@@ -236,3 +236,41 @@ def test_autotune_keeps_state_and_plan_runs_after_it():
     g.set_plan(0)
     g.time_loop(2)
     assert np.array_equal(g.populations(), want)
+
+
+@pytest.mark.parametrize("precision", [LB3D_FP64, LB3D_FP32_DEV])
+def test_lees_edwards_edge_columns_follow_every_state_change(precision):
+    """On the two-pass plan the Lees-Edwards halo planes read compact copies of
+    the edge columns that the moments and collide passes write.  Every way the
+    state changes outside those passes (one-pass steps, the staged surface,
+    the tuner's trials, profiled steps, an upload) must leave the run bitwise
+    equal to the straight two-pass run."""
+    kw = dict(ncomp=2, precision=precision, g_br=0.09, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.04)
+    dims = (20, 9, 12)
+    _, ref = pair(*dims, **kw)
+    ref.set_plan(0)
+    ref.time_loop(12)
+    want = ref.populations()
+    mode, tx, ty = fused_shapes(precision)[0]
+    o, g = pair(*dims, **kw)
+    g.set_plan(0)
+    g.time_loop(2)                 # graph-replayed two-pass steps
+    g.set_plan(mode, tx, ty, 4)
+    g.time_loop(3)                 # one-pass steps: the edge copies are not written
+    g.set_plan(0)
+    g.time_step()                  # a staged two-pass step after them
+    g.time_loop(1)
+    g.autotune(1)                  # the trials overwrite the edge copies
+    g.set_plan(0)
+    g.time_loop(2)
+    g.profile_steps(2)
+    g.time_loop(1)
+    assert g.nt == 12
+    got = g.populations()
+    assert np.array_equal(got, want), f"max {np.abs(got - want).max()}"
+    # an upload replaces the state under the edge copies of the last step
+    g.upload_sites(o.N)
+    g.bc_init()
+    g.time_loop(12)
+    got = g.populations()
+    assert np.array_equal(got, want), f"after upload: max {np.abs(got - want).max()}"
