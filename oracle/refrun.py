@@ -50,22 +50,29 @@ def record_length(ternary):
 
 def namelist(name, dims, steps, ternary, restore=False, n_checkpoint=None, init_rho=0.7, lees_edwards=False,
              g_br=0.08, g_bs=-0.006, g_ss=-0.003, tau_d=1.0, beta=10.0, d_0=1.0, bdist=2, shear_u=0.0,
-             tau=(1.0, 1.0, 1.0), amass=(1.0, 1.0, 1.0), seed=1):
+             tau=(1.0, 1.0, 1.0), amass=(1.0, 1.0, 1.0), seed=1, g_accn=(0.0, 0.0, 0.0), g_accn_min=(0, 0, 0),
+             g_accn_max=(0, 0, 0), inv_fluid=0, inv_type=0, inv_flux=(0.0, 0.0, 0.0)):
     """The reference's input file: `steps` timesteps of a dims lattice, either
     restored from ./checkpoint/checkpoint_<name>_t00000000-0000000000_p000000.bin
     ('ori' format, init_cond 7) or started from INIT_RAND (init_cond -1,
     mean density init_rho per component); no dumps; a checkpoint every
-    n_checkpoint steps when given."""
+    n_checkpoint steps when given.  The physics arguments are those of
+    oracle.pyoracle.Oracle: g_accn, g_accn_min and g_accn_max are (x, y, z)
+    (the reference's g_accn_x, g_accn_y, g_accn and their box bounds, 1-based,
+    0 for the lattice's extent), inv_fluid 12 with inv_type and the fluxes
+    inv_flux = (pr, pb, pg) is the Zou-He boundary on the z planes."""
     nx, ny, nz = dims
     f = 2.0 * init_rho  # INIT_RAND draws each population in [0, f g_i^-1 / 19]
     ck = n_checkpoint if n_checkpoint else 10 * steps + 10
+    zou_he = inv_fluid == 12
+    inv = 5 if lees_edwards else (12 if zou_he else -1)
     return f"""&fixed_input
 nx = {nx}
 ny = {ny}
 nz = {nz}
 seed = {seed}
 obs_file = 'empty.dat'
-boundary_cond = {-1 if lees_edwards else 0}
+boundary_cond = {-1 if (lees_edwards or zou_he) else 0}
 /
 
 &variable_input
@@ -89,7 +96,11 @@ init_cond = {7 if restore else -1}
 fr = {f}
 fb = {f}
 fg = {f if ternary else 0.0}
-inv_fluid = {5 if lees_edwards else -1}
+inv_fluid = {inv}
+inv_type = {inv_type}
+pr = {inv_flux[0]}
+pb = {inv_flux[1]}
+pg = {inv_flux[2]}
 beta = {beta}
 /
 
@@ -109,7 +120,15 @@ bdist = {bdist}
 perturbation = 0
 shear_u = {shear_u}
 shear_omega = 0
-g_accn = 0.0
+g_accn = {g_accn[2]}
+g_accn_x = {g_accn[0]}
+g_accn_y = {g_accn[1]}
+g_accn_min = {g_accn_min[2]}
+g_accn_min_x = {g_accn_min[0]}
+g_accn_min_y = {g_accn_min[1]}
+g_accn_max = {g_accn_max[2]}
+g_accn_max_x = {g_accn_max[0]}
+g_accn_max_y = {g_accn_max[1]}
 n_checkpoint = {ck}
 restore_string = 't00000000-0000000000'
 num_chkp_files = 0

@@ -34,14 +34,13 @@ def ref():
 @pytest.mark.parametrize("name", list(make_ref_fp64.CASES))
 def test_gpu_fp64_matches_reference(ref, name, plan):
     s = json.loads(ref[f"{name}/spec"].tobytes().decode())
-    dims, ternary, shear = tuple(s["dims"]), s["ternary"], s["shear"]
-    o = make_ref_fp64.initial_oracle(dims, ternary, shear)
-    kw = dict(ncomp=3 if ternary else 2, g_br=0.08, bdist=2, restore=1, shear_sum=0.0)
-    if ternary:
-        kw.update(make_ref_fp64.TERNARY_KW)
+    c = make_ref_fp64.CASES[name]
+    ternary, shear = c["ternary"], c["shear"]
+    o = make_ref_fp64.initial_oracle(c)
+    kw = dict(c["phys"], ncomp=3 if ternary else 2, restore=1, shear_sum=0.0)
     if shear is not None:
         kw.update(boundary=LB3D_BC_LEES_EDWARDS, shear_u=shear)
-    g = Lattice(*dims, precision=LB3D_FP64, **kw)
+    g = Lattice(*c["dims"], precision=LB3D_FP64, **kw)
     g.upload_sites(o.N)
     g.bc_init()
     if plan == "two-pass":
@@ -58,11 +57,11 @@ def test_gpu_fp64_matches_reference(ref, name, plan):
         assert d <= 1e-11
     else:
         rho = g.density()
-        for c, obs in enumerate(("od", "wd")):
-            d = np.abs(rho[c] - ref[f"{name}/{obs}"]).max()
-            print(f"{name} {plan}: max |rho_{c} - reference| {d:.3e}")
+        for k, obs in enumerate(("od", "wd")):
+            d = np.abs(rho[k] - ref[f"{name}/{obs}"]).max()
+            print(f"{name} {plan}: max |rho_{k} - reference| {d:.3e}")
             assert d <= 1e-11
-    m0 = ref[f"{name}/mass"][0]
+    m1 = ref[f"{name}/mass"][1]
     m = g.sanity_check()[0]
-    assert np.all(np.abs(m[:nc] - m0[:nc]) <= 1e-13 * np.abs(m0[:nc]))
+    assert np.all(np.abs(m[:nc] - m1[:nc]) <= 1e-13 * np.abs(m1[:nc]))
     g.close()

@@ -3,10 +3,13 @@
 tests/golden/reference_fp64.npz holds the reference's state after 100
 timesteps from the oracle's seeded initial states (written by
 tests/golden/make_ref_fp64.py, which runs the reference built from its own
-sources by oracle/ref_build.sh).  The oracle, from the same initial state,
-must match it to 1e-11 in every population, dipole and density (it matches
-bit for bit where the fixtures were made), and both must conserve each
-component's mass to 1e-13 relative.
+sources by oracle/ref_build.sh): periodic binary and ternary lattices,
+Lees-Edwards with both shear signs, every other equilibrium form with a
+body-force box, unequal relaxation times and masses, and the Zou-He flux
+boundary.  The oracle, from the same initial state, must match it to 1e-11 in
+every population, dipole and density (it matches bit for bit where the
+fixtures were made) and in each component's mass to 1e-13 relative, and the
+reference must conserve mass to 1e-13 where no boundary lets it in or out.
 """
 import importlib.util
 import json
@@ -37,8 +40,10 @@ def spec(ref, name):
 @pytest.mark.parametrize("name", list(CASES))
 def test_oracle_matches_reference_fp64(ref, name):
     s = spec(ref, name)
-    dims, ternary, shear = tuple(s["dims"]), s["ternary"], s["shear"]
-    o = initial_oracle(dims, ternary, shear)
+    c = CASES[name]
+    assert tuple(s["dims"]) == c["dims"] and s["phys"] == json.loads(json.dumps(c["phys"])), "fixture out of date"
+    ternary = c["ternary"]
+    o = initial_oracle(c)
     o.bc_init()
     o.run(s["steps"])
     assert o.nt == s["steps"]
@@ -56,20 +61,28 @@ def test_oracle_matches_reference_fp64(ref, name):
             assert d <= 1e-11
     m0, m1 = ref[f"{name}/mass"]
     nc = 3 if ternary else 2
-    assert np.all(np.abs(m1[:nc] - m0[:nc]) <= 1e-13 * np.abs(m0[:nc])), "reference mass"
     mo = o.sanity()[0]
-    assert np.all(np.abs(mo[:nc] - m0[:nc]) <= 1e-13 * np.abs(m0[:nc])), "oracle mass"
+    assert np.all(np.abs(mo[:nc] - m1[:nc]) <= 1e-13 * np.abs(m1[:nc])), "oracle mass against the reference's"
+    # the Zou-He planes let mass in and out; Guo's fourth-order form (bdist 5)
+    # with a body force does not keep the mass of the site (the reference's
+    # run: 3e-7 relative in 100 steps)
+    if c["phys"].get("inv_fluid", 0) != 12 and c["phys"]["bdist"] != 5:
+        assert np.all(np.abs(m1[:nc] - m0[:nc]) <= 1e-13 * np.abs(m0[:nc])), "reference mass conservation"
 
 
 def test_fixture_cases_differ_from_the_initial_state(ref):
-    """100 steps move every case well away from its initial state, and the
-    fixtures of the two shear signs differ: a fixture equal to its input
-    would pin nothing."""
-    for name, (dims, ternary, shear, kind) in CASES.items():
-        if kind != "records":
+    """100 steps move every case well away from its initial state, the
+    fixtures of the two shear signs differ, and so do the equilibrium forms:
+    a fixture equal to its input, or to another case's, would pin nothing."""
+    for name, c in CASES.items():
+        if c["kind"] != "records":
             continue
-        o = initial_oracle(dims, ternary, shear)
-        start = refrun.records_from_sites(o.interior(), ternary)
-        assert np.abs(ref[f"{name}/out"] - start).max() > 1e-4
+        o = initial_oracle(c)
+        start = refrun.records_from_sites(o.interior(), c["ternary"])
+        assert np.abs(ref[f"{name}/out"] - start).max() > 1e-5, name
     a, b = ref["le_binary_16x8x8/out"], ref["le_binary_16x8x8_negative/out"]
     assert np.abs(a - b).max() > 1e-4
+    forms = [ref[f"binary_bdist{k}_body/out"] for k in (3, 4, 5, 6)]
+    for i in range(len(forms)):
+        for j in range(i + 1, len(forms)):
+            assert np.abs(forms[i] - forms[j]).max() > 1e-9, (i, j)
