@@ -267,7 +267,8 @@ int lb3d_gpu_step_noalloc(const lb3d_params *p, const void *f_in, void *f_out, c
  * per job).  Many independent periodic lattices of uneven sizes -- parameter
  * sweeps, seeds -- advanced together: each timestep is one persistent launch
  * per phase over all lattices, replayed from a captured HIP graph.  All
- * lattices share ncomp and precision; every other parameter is per lattice.
+ * lattices share ncomp and precision; every other parameter is per lattice
+ * (bdist 0-6, body force; no walls or invasion).
  */
 typedef struct lb3d_batch lb3d_batch;
 

@@ -177,7 +177,7 @@ int build_cm(lb3d_batch *b, int cols, int lz, int threads)
 {
     if (threads != 256 && threads != 512) return set_error(LB3D_EINVAL, "tile_threads must be 256 or 512");
     if (!b->B->cm_supported(b->C))
-        return set_error(LB3D_EINVAL, "the collide-moments plan runs FP32 ternary batches");
+        return set_error(LB3D_EINVAL, "the collide-moments plan runs FP32 ternary batches of bdist 0 and 2");
     if (cols < 1 || cols > threads - 2 || lz < 1)
         return set_error(LB3D_EINVAL, "need 1 <= tile_cols <= tile_threads - 2 and lz >= 1");
     b->cm_cols = 0;
@@ -296,7 +296,7 @@ int lb3d_gpu_batch_create(int32_t nlat, const lb3d_params *params, lb3d_batch **
             return set_error(LB3D_EINVAL, "all lattices of a batch share ncomp and precision");
         if (q.boundary != LB3D_BC_PERIODIC) return set_error(LB3D_EINVAL, "batched lattices are periodic");
         if (q.ncomp != 2 && q.ncomp != 3) return set_error(LB3D_EINVAL, "ncomp must be 2 or 3");
-        if (q.bdist != 0 && q.bdist != 2) return set_error(LB3D_EINVAL, "bdist must be 0 or 2");
+        if (q.bdist < 0 || q.bdist > 6) return set_error(LB3D_EINVAL, "bdist must be 0..6");
         if (q.inv_fluid != 0) return set_error(LB3D_EINVAL, "the batch runs lattices without invasion");
         if (q.precision != LB3D_FP64 && q.precision != LB3D_FP32_DEV)
             return set_error(LB3D_EINVAL, "precision must be LB3D_FP64 or LB3D_FP32_DEV");

@@ -1728,6 +1728,17 @@ __device__ __forceinline__ uint2 batch_item(const uint2 *__restrict__ items, uin
     return w;
 }
 
+// The batch's collide launches one kernel per equilibrium form present: the
+// two compile-time forms bdist 0 and 2, and BD_GENERAL for the others.
+template <int BD>
+__device__ __forceinline__ bool batch_form_is(int bdist)
+{
+    if constexpr (BD == BD_GENERAL)
+        return bdist != 0 && bdist != 2;
+    else
+        return bdist == BD;
+}
+
 template <typename R, bool Dev, int C>
 __global__ __launch_bounds__(256) void k_batch_moments(const BatchLat<R> *__restrict__ lat,
                                                        const uint2 *__restrict__ items,
@@ -1782,7 +1793,7 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         uint32_t s = w.y + threadIdx.x;
         // non-temporal stores of the populations (NT): -2% per step of the
         // 48-lattice ternary ensemble; non-temporal loads were slower
-        if (s < L.P.ns && L.P.bdist == BD)
+        if (s < L.P.ns && batch_form_is<BD>(L.P.bdist))
             collide_site<R, Dev, C, false, BD, false, false, NT ? 2 : 0>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
                                                                C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
     }

@@ -531,7 +531,7 @@ class LauncherT : public Launcher {
 template <typename R, bool Dev>
 class BatchLauncherT : public BatchLauncher {
   public:
-    int bd_mask = 0;  // equilibrium forms present in the batch (bit 0: bdist 0, bit 2: bdist 2)
+    int bd_mask = 0;  // equilibrium forms present in the batch (bit b: bdist b)
     bool nt = false;  // non-temporal population stores: the populations exceed the memory-side cache
     void set_bdist_mask(int m) override { bd_mask = m; }
     void set_pop_bytes(uint64_t bytes) override { nt = bytes > MALL_BYTES; }
@@ -559,9 +559,10 @@ class BatchLauncherT : public BatchLauncher {
         } else if (ph == 1) {
             if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db, xg);
         } else {
-            // every lattice's collide runs in the launch of its own equilibrium form
-            for (int bd : {0, 2}) {
-                if (!(bd_mask & (1 << bd))) continue;
+            // every lattice's collide runs in the launch of its own equilibrium
+            // form: bdist 0, bdist 2, or one launch for all the others
+            for (int bd : {0, 2, BD_GENERAL}) {
+                if (!(bd == BD_GENERAL ? (bd_mask & ~0b101) : (bd_mask & (1 << bd)))) continue;
                 if (nt)
                     collide_bd<true>(bd, L, it, nitems, nwg, f, fout, aux, db, ncomp, s);
                 else
@@ -569,7 +570,8 @@ class BatchLauncherT : public BatchLauncher {
             }
         }
     }
-    bool cm_supported(int ncomp) const override { return Dev && ncomp == 3; }
+    // FP32 ternary batches whose lattices all take bdist 0 or 2
+    bool cm_supported(int ncomp) const override { return Dev && ncomp == 3 && !(bd_mask & ~0b101); }
     void cm_collide(const void *lat, const void *items, uint32_t nitems, unsigned nwg, int threads, const void *f,
                     void *fout, const void *aux_in, void *aux_out, const void *db, hipStream_t s) override
     {
@@ -611,6 +613,12 @@ class BatchLauncherT : public BatchLauncher {
         if (ncomp == 3 && bd == 0)
             k_batch_collide<R, Dev, 3, 0, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
                                                                      (const R *)db, (R *)fout, xg);
+        else if (ncomp == 3 && bd == BD_GENERAL)
+            k_batch_collide<R, Dev, 3, BD_GENERAL, NT><<<nwg, BLOCK, 0, s>>>(
+                L, it, nitems, (const R *)f, (const R *)aux, (const R *)db, (R *)fout, xg);
+        else if (bd == BD_GENERAL)
+            k_batch_collide<R, Dev, 2, BD_GENERAL, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f,
+                                                                              (const R *)aux, nullptr, (R *)fout, xg);
         else if (ncomp == 3)
             k_batch_collide<R, Dev, 3, 2, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
                                                                      (const R *)db, (R *)fout, xg);

@@ -46,13 +46,13 @@ def test_invalid_parameters_rejected(kw, msg):
 
 @pytest.mark.parametrize("kw,msg", [
     (dict(inv_fluid=12), "invasion"),
-    (dict(bdist=4), "bdist"),
+    (dict(bdist=7), "bdist"),
     (dict(boundary=gpu.LB3D_BC_LEES_EDWARDS), "periodic"),
     (dict(precision=gpu.LB3D_FP32_DEV), "share"),
 ])
 def test_batch_invalid_parameters_rejected(kw, msg):
-    """The batch takes periodic lattices of one form with bdist 0 or 2 and no
-    invasion; the checks run before any device call."""
+    """The batch takes periodic lattices of one form (ncomp, precision) with
+    bdist 0-6 and no invasion; the checks run before any device call."""
     with pytest.raises(gpu.LB3DError) as e:
         gpu.Batch([(8, 8, 8), (6, 6, 6)], per_lattice=[{}, kw], ncomp=2)
     assert e.value.code == 1 and msg in str(e.value)

@@ -174,3 +174,47 @@ def test_collide_moments_plan_switches_and_reinit():
     assert b.nt == 6  # init_noise restarts the batch's step count
     for k in range(len(CM_DIMS)):
         np.testing.assert_array_equal(b.populations(k), ref.populations(k), err_msg=f"lattice {k}")
+
+
+@pytest.mark.parametrize("ncomp,precision", [(2, LB3D_FP64), (3, LB3D_FP32_DEV), (3, LB3D_FP64)])
+def test_batch_every_equilibrium_form_with_body_force(ncomp, precision):
+    """Lattices of every equilibrium form of code/lb3d_bdist.F90 in one batch,
+    with unequal relaxation times and a body-force box (Guo's term of bdist 4
+    and 5 on inside it): the forms other than 0 and 2 share one collide
+    launch; every lattice equals its single-lattice run bitwise, on the
+    persistent, grouped and sequential plans, and the FP64 ones the oracle."""
+    forms = [0, 1, 2, 3, 4, 5, 6]
+    dims = [(8, 8, 8), (13, 7, 5), (16, 12, 9), (5, 9, 11), (10, 6, 7), (7, 7, 7), (9, 5, 6)]
+    per = [dict(lattice_kw(k, ncomp), bdist=forms[k], tau=(0.9 + 0.02 * k, 1.1, 1.0),
+                g_accn=(2e-4, -1e-4, 3e-4), g_accn_min=(2, 2, 2), g_accn_max=(4, 5, 4)) for k in range(len(forms))]
+
+    def make():
+        b = Batch(dims, per_lattice=per, ncomp=ncomp, precision=precision)
+        for k in range(len(dims)):
+            b.init_noise(k, 60 + k, amp=(0.02, 0.02, 0.01))
+        return b
+
+    plans = [make(), make(), make()]
+    plans[0].set_plan(0, 64, 1)
+    plans[1].set_plan(0, 64, 1)
+    plans[1].set_groups(1)
+    plans[2].set_plan(1)
+    for b in plans:
+        b.time_loop(7)
+    for k, d in enumerate(dims):
+        g = Lattice(*d, ncomp=ncomp, precision=precision, **per[k])
+        g.init_noise(60 + k, amp=(0.02, 0.02, 0.01))
+        g.bc_init()
+        g.time_loop(7)
+        want = g.populations()
+        for j, b in enumerate(plans):
+            np.testing.assert_array_equal(b.populations(k), want, err_msg=f"plan {j} lattice {k} bdist {forms[k]}")
+        g.close()
+        if precision == LB3D_FP64:
+            o = Oracle(*d, ncomp=ncomp, **per[k])
+            o.init_noise(60 + k, amp=(0.02, 0.02, 0.01))
+            o.bc_init()
+            o.run(7)
+            assert np.abs(plans[0].populations(k) - o.populations()).max() <= 1e-11
+    for b in plans:
+        b.close()
