@@ -709,19 +709,25 @@ int lb3d_gpu_collision(lb3d_ctx *c)
 // One whole timestep on the stream (graph-capturable: kernels only).
 static int enqueue_step(lb3d_ctx *c)
 {
-    // (Lees-Edwards on the two-pass plan: the edge columns are valid, see
-    // ensure_edges before the steps are enqueued)
+    // Lees-Edwards, two-pass plan: the halo-plane kernels compute the step's
+    // offset state from the stored one and the collide stores it (no launch
+    // of k_le_neighbours); the edge columns are valid (ensure_edges before
+    // the steps are enqueued).  One-pass plan: k_le_neighbours, then the
+    // halo planes from the lattice.
+    const bool merged = c->le && !c->plan.fused;
     if (c->le) {
-        launch_le_neighbours(c, -1);
-        c->L->le_faces(c->b, false, compact_edges(c), c->stream);
+        if (!merged) launch_le_neighbours(c, -1);
+        c->L->le_faces(c->b, false, compact_edges(c), c->stream, merged ? -1 : LE_STATE_AS_IS);
     }
     if (c->plan.fused) {
         if (c->le) c->L->le_faces(c->b, true, false, c->stream);
     } else {
         c->L->moments(c->b, c->stream);
-        if (c->le) c->L->le_faces(c->b, true, true, c->stream);
+        if (c->le) c->L->le_faces(c->b, true, true, c->stream, -1);
     }
+    if (merged) c->L->set_le_commit(-1);
     int rc = launch_collision(c);
+    if (merged) c->L->set_le_commit(LE_STATE_AS_IS);
     if (rc) return rc;
     flip(c);
     return LB3D_OK;
@@ -1323,15 +1329,21 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
     HIP_TRY(hipEventRecord(t0, c->stream));
     for (int st = 0; st < nsteps; st++) {
         auto mark = [&](int k) { used[(size_t)st * K_COUNT + k] = 1; };
+        // as enqueue_step: the two-pass plan advances the Lees-Edwards state
+        // in its halo-plane kernels and collide, the one-pass plan launches
+        // k_le_neighbours
+        const bool merged = c->le && !c->plan.fused;
         if (c->le) {
-            HIP_TRY(rec(st, K_LEN, true));
-            launch_le_neighbours(c, -1);
-            HIP_TRY(rec(st, K_LEN, false));
+            if (!merged) {
+                HIP_TRY(rec(st, K_LEN, true));
+                launch_le_neighbours(c, -1);
+                HIP_TRY(rec(st, K_LEN, false));
+                mark(K_LEN);
+            }
             HIP_TRY(rec(st, K_LE6, true));
             ensure_edges(c);
-            c->L->le_faces(c->b, false, compact_edges(c), c->stream);
+            c->L->le_faces(c->b, false, compact_edges(c), c->stream, merged ? -1 : LE_STATE_AS_IS);
             HIP_TRY(rec(st, K_LE6, false));
-            mark(K_LEN);
             mark(K_LE6);
         }
         if (!c->plan.fused) {
@@ -1342,7 +1354,7 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
         }
         if (c->le) {
             HIP_TRY(rec(st, K_LE8, true));
-            c->L->le_faces(c->b, true, compact_edges(c), c->stream);
+            c->L->le_faces(c->b, true, compact_edges(c), c->stream, merged ? -1 : LE_STATE_AS_IS);
             HIP_TRY(rec(st, K_LE8, false));
             mark(K_LE8);
         }
@@ -1360,7 +1372,9 @@ int lb3d_gpu_profile_steps(lb3d_ctx *c, int32_t nsteps, int32_t max_kernels, con
                 mark(K_DIP);
             }
             HIP_TRY(rec(st, K_COL, true));
+            if (merged) c->L->set_le_commit(-1);
             c->L->collide(c->b, c->stream);
+            if (merged) c->L->set_le_commit(LE_STATE_AS_IS);
             HIP_TRY(rec(st, K_COL, false));
             mark(K_COL);
         }

@@ -86,6 +86,23 @@ __device__ __forceinline__ uint32_t sidx(int x, int y, int z, int nx, int ny)
     return (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
 }
 
+// The Lees-Edwards state of the next step (le_neighbours with the stage-6
+// shear update, code/lb3d_bc_leesedwards.F90:86-140, code/lb3d_bc.F90:110):
+// nt_set >= 0 sets the step counter, -1 advances it by one.  FP64.
+__device__ __forceinline__ LEState le_advance(LEState s, int nz, int nt_set)
+{
+    s.nt = nt_set >= 0 ? nt_set : s.nt + 1;
+    // (cos(0) = 1 exactly: a steady shear skips the cosine)
+    s.shear_u = s.shear_omega == 0.0 ? s.shear_tmp : s.shear_tmp * cos(s.shear_omega * (double)s.nt);
+    s.shear_sum = s.shear_sum + s.shear_u;
+    // lelo = mod(int(2.*shear_sum), nz): truncation, sign of the dividend
+    long t = (long)(2.0 * s.shear_sum);
+    s.lelo = (int)(t % (long)nz);
+    double lf = 2.0 * s.shear_sum;
+    s.lefrac = fabs(lf - trunc(lf));
+    return s;
+}
+
 // Loads and stores through a wave-uniform plane base plus a 32-bit byte offset
 // per lane: the plane bases live in SGPRs and each access needs one VGPR for
 // its offset (the global "saddr" addressing form) instead of a 64-bit address.
@@ -1059,8 +1076,14 @@ __global__ __launch_bounds__(256) void k_collide(Phys<R> P, const R *__restrict_
                                                  const R *__restrict__ face_aux,
                                                  const R *__restrict__ db, R *__restrict__ fout,
                                                  uint32_t xg, const uint8_t *__restrict__ rock,
-                                                 R *__restrict__ edge_pc = nullptr)
+                                                 R *__restrict__ edge_pc = nullptr,
+                                                 LEState *__restrict__ le_commit = nullptr, int le_adv = 0)
 {
+    // Lees-Edwards: the step's state, which its halo-plane kernels computed
+    // from the stored one, replaces it (no kernel of the step reads it after
+    // them)
+    if constexpr (LE)
+        if (le_commit && blockIdx.x == 0 && threadIdx.x == 0) *le_commit = le_advance(*le_commit, P.nz, le_adv);
     uint32_t s = xcd_tile(blockIdx.x, gridDim.x, xg) * blockDim.x + threadIdx.x;
     if (s >= P.ns) return;
     // measured on MI355X (FP32 and FP64): ternary loads and stores (128^3:
@@ -1622,11 +1645,13 @@ __device__ __forceinline__ int wrapz1(int k, int n)
 constexpr int LE_SITES = 64;
 // Compact: the sources come from the edge columns (edge: the pre-advection
 // ones, or for Post the streamed ones) instead of the lattice.
+// adv: LE_STATE_AS_IS, or the step's state is le_advance(*Lp, nz, adv) (the
+// collide of the step stores it; every block computes it alike)
 template <typename R, bool Dev, bool Post, bool Compact = false>
 __global__ __launch_bounds__(2 * LE_SITES) void k_le_faces(Phys<R> P, const LEState *__restrict__ Lp,
                                                            const R *__restrict__ f, R *__restrict__ face_pc,
                                                            R *__restrict__ face_aux,
-                                                           const R *__restrict__ edge = nullptr)
+                                                           const R *__restrict__ edge, int adv)
 {
     __shared__ R lo_done[2][19][LE_SITES];  // the corrected low-plane sites
     const uint32_t plane = (uint32_t)P.ny * (uint32_t)P.nz;
@@ -1634,7 +1659,7 @@ __global__ __launch_bounds__(2 * LE_SITES) void k_le_faces(Phys<R> P, const LESt
     const bool low = threadIdx.x < LE_SITES;  // wave-uniform
     const uint32_t t = blockIdx.x * LE_SITES + j;
     const bool live = t < plane;
-    const LEState L = *Lp;
+    const LEState L = adv == LE_STATE_AS_IS ? *Lp : le_advance(*Lp, P.nz, adv);
     const int y = live ? (int)(t % (uint32_t)P.ny) : 0, z = live ? (int)(t / (uint32_t)P.ny) : 0;  // 0-based
     const int z1 = z + 1;                                                                           // 1-based
     const int lelo = L.lelo, bsh = lelo >= 0 ? 0 : -1;
@@ -2036,16 +2061,7 @@ __global__ __launch_bounds__(256) void k_batch_edge_moments(const BatchLat<R> *_
 static __global__ void k_le_neighbours(LEState *L, int nz, int nt_set)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    LEState s = *L;
-    s.nt = nt_set >= 0 ? nt_set : s.nt + 1;
-    s.shear_u = s.shear_tmp * cos(s.shear_omega * (double)s.nt);
-    s.shear_sum = s.shear_sum + s.shear_u;
-    // lelo = mod(int(2.*shear_sum), nz): truncation, sign of the dividend
-    long t = (long)(2.0 * s.shear_sum);
-    s.lelo = (int)(t % (long)nz);
-    double lf = 2.0 * s.shear_sum;
-    s.lefrac = fabs(lf - trunc(lf));
-    *L = s;
+    *L = le_advance(*L, nz, nt_set);
 }
 
 }  // namespace lb3d

@@ -58,6 +58,10 @@ struct LEState {
     int lelo;
     int nt;
 };
+// adv argument of the Lees-Edwards halo-plane kernels and of the collide's
+// state update: the stored state is the step's own (k_le_neighbours has
+// advanced it); any other value is the nt_set of le_advance
+constexpr int LE_STATE_AS_IS = -2;
 
 // The other equilibrium forms of code/lb3d_bdist.F90 (COMPAT_BDIST): 3 second
 // order in (u, du) (:372-391), 4 and 5 Guo's second and fourth order with the

@@ -348,7 +348,7 @@ class LauncherT : public Launcher {
         else if (le)
             k_collide<R, Dev, 2, true, BD, false, false, NT><<<tiles(), BLOCK, 0, s>>>(
                 P, f, (const R *)b.face_pc, (const R *)b.aux, (const R *)b.face_aux, nullptr, fo, xg, nullptr,
-                (R *)b.edge_pc);
+                (R *)b.edge_pc, le_commit == LE_STATE_AS_IS ? nullptr : b.le, le_commit);
         else if (b.rock)
             k_collide<R, Dev, 2, false, BD, true><<<tiles(), BLOCK, 0, s>>>(P, f, nullptr, (const R *)b.aux,
                                                                             nullptr, nullptr, fo, xg, b.rock);
@@ -369,21 +369,25 @@ class LauncherT : public Launcher {
         else
             n ? collide_bd<BD_GENERAL, true>(b, s) : collide_bd<BD_GENERAL, false>(b, s);
     }
-    void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s) override
+    void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s, int adv) override
     {
         unsigned plane = (unsigned)P.ny * (unsigned)P.nz;
         unsigned nb = (plane + LE_SITES - 1) / LE_SITES;
         const R *f = (const R *)b.f[b.cur];
         R *fp = (R *)b.face_pc, *fa = (R *)b.face_aux;
         if (post && compact)
-            k_le_faces<R, Dev, true, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_adv);
+            k_le_faces<R, Dev, true, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_adv, adv);
         else if (post)
-            k_le_faces<R, Dev, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa);
+            k_le_faces<R, Dev, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, nullptr, adv);
         else if (compact)
-            k_le_faces<R, Dev, false, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_pc);
+            k_le_faces<R, Dev, false, true><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, (const R *)b.edge_pc, adv);
         else
-            k_le_faces<R, Dev, false><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa);
+            k_le_faces<R, Dev, false><<<nb, 2 * LE_SITES, 0, s>>>(P, b.le, f, fp, fa, nullptr, adv);
     }
+    // the next two-pass collide stores le_advance(state, adv) unless adv is
+    // LE_STATE_AS_IS (then it leaves the state alone)
+    int le_commit = LE_STATE_AS_IS;
+    void set_le_commit(int adv) override { le_commit = adv; }
     void le_gather_edges(const DevBufs &b, hipStream_t s) override
     {
         unsigned n = 2u * 2u * 19u * (unsigned)P.ny * (unsigned)P.nz;

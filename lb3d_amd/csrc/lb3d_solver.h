@@ -44,7 +44,11 @@ class Launcher {
     // Lees-Edwards x-halo planes, pre- (post = false) or post-advection;
     // compact: from the edge columns (b.edge_pc written by the two-pass
     // collide or le_gather_edges, b.edge_adv by the moments pass)
-    virtual void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s) = 0;
+    // adv: LE_STATE_AS_IS, or the halo-plane kernels compute the step's state
+    // from the stored one (le_advance with nt_set = adv) and the next collide
+    // stores it (set_le_commit)
+    virtual void le_faces(const DevBufs &b, bool post, bool compact, hipStream_t s, int adv = LE_STATE_AS_IS) = 0;
+    virtual void set_le_commit(int adv) = 0;
     // b.edge_pc from the populations f[cur]
     virtual void le_gather_edges(const DevBufs &b, hipStream_t s) = 0;
     // the Shan-Chen force field from b.aux and b.db into out[(c*3 + k)*ns + s]
