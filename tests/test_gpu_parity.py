@@ -92,6 +92,35 @@ def test_lees_edwards_binary_fp64():
     assert (lg, sg, fg, ug) == (lo, so, fo, uo)
 
 
+def test_lees_edwards_offset_state_across_step_paths():
+    """The Lees-Edwards offset state advances once per step whichever path ran
+    the step: staged steps (one kernel updates it), graph-replayed and
+    profiled two-pass steps (the halo-plane kernels compute it, the collide
+    stores it), the tuner's trials (they leave it unchanged) and a one-pass
+    step; populations and (shear_sum, lelo, lefrac, shear_u) as the oracle's."""
+    o, g = pair(16, 12, 20, ncomp=2, g_br=0.08, boundary=LB3D_BC_LEES_EDWARDS, shear_u=0.07)
+    g.set_plan(0)
+    for _ in range(3):
+        g.time_step()
+    g.time_loop(5)
+    g.profile_steps(2)
+    g.autotune(2)
+    for mode, tx, ty in ((3, 32, 8), (2, 32, 8), (1, 32, 8), (3, 64, 4)):
+        try:
+            g.set_plan(mode, tx, ty, 8)
+            break
+        except LB3DError:
+            continue
+    else:
+        raise AssertionError("no one-pass plan for this lattice")
+    g.time_loop(1)
+    g.set_plan(0)
+    g.time_loop(3)
+    o.run(14)
+    assert_pops_close(o, g, what="LE across step paths x14")
+    assert g.le_state() == o.le_state()
+
+
 def test_lees_edwards_negative_shear_and_clamp():
     """Negative shear (the lelo < 0 branch) and a strong shear that makes the
     Galilean correction trip the negativity clamp."""
