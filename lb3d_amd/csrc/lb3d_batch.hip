@@ -103,7 +103,7 @@ void group_phase(lb3d_batch *b, size_t g, int ph)
     const uint32_t i0 = b->groups[g], n = b->groups[g + 1] - i0;
     const unsigned nwg = b->nwg < xcd_round(n) ? b->nwg : xcd_round(n);
     b->B->phase(ph, b->lat, (const uint2 *)b->items + i0, n, nwg, b->f[b->cur], b->f[1 - b->cur], b->aux, b->db,
-                b->C, b->stream);
+                b->C, b->stream, true);
 }
 
 void *aux_of(const lb3d_batch *b, int j) { return j == 0 ? b->aux : b->aux1; }
@@ -113,7 +113,7 @@ void *aux_of(const lb3d_batch *b, int j) { return j == 0 ? b->aux : b->aux1; }
 void prime_aux(lb3d_batch *b)
 {
     b->B->phase(0, b->lat, b->items, b->nitems, xcd_round(b->nitems), b->f[b->cur], b->f[1 - b->cur],
-                aux_of(b, b->cur), b->db, b->C, b->stream);
+                aux_of(b, b->cur), b->db, b->C, b->stream, false);
     b->aux_ok = true;
 }
 
@@ -126,7 +126,7 @@ void cm_phase(lb3d_batch *b, int ph)
     const int c = b->cur;
     if (ph == 1)
         b->B->phase(1, b->lat, b->items, b->nitems, xcd_round(b->nitems), b->f[c], b->f[1 - c], aux_of(b, c), b->db,
-                    b->C, b->stream);
+                    b->C, b->stream, false);
     else if (ph == 2)
         b->B->cm_collide(b->lat, b->cm_items, b->cm_nitems, xcd_round(b->cm_nitems), b->cm_threads, b->f[c],
                          b->f[1 - c], aux_of(b, c), aux_of(b, 1 - c), b->db, b->stream);

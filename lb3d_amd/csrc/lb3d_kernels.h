@@ -458,7 +458,10 @@ __device__ __forceinline__ void moments_core(const Phys<R> &P, const R *__restri
     da[2] = da2;
 }
 
-template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false>
+// PK (ternary, the batch's three-pass plan): the packed aux layout, one
+// 4-vector (psi_r, psi_b, psi_s, rho_s) per site in place of the six planes,
+// so that the force stencil loads one vector per neighbour
+template <typename R, bool Dev, int C, bool LE, bool Rock = false, bool ZH = false, bool PK = false>
 __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ db, R *__restrict__ aux,
@@ -469,10 +472,15 @@ __device__ __forceinline__ void moments_site(const Phys<R> &P, const R *__restri
     site_xyz(s, P.nx, P.ny, x, y, z);
     R rho[C], da[3];
     moments_core<R, Dev, C, LE, Rock, ZH>(P, f, face_pc, db, x, y, z, rho, da, rock, edge_adv);
+    if constexpr (PK && C == 3) {
+        ((V4<R> *)aux)[s] = V4<R>{psi_of<Dev>(rho[0], P.emr0[0]), psi_of<Dev>(rho[1], P.emr0[1]),
+                                  psi_of<Dev>(rho[2], P.emr0[2]), rho[2]};
+    } else {
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
-        st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
+        for (int c = 0; c < C; c++) {
+            st_plane(aux + (size_t)c * P.ns, s, psi_of<Dev>(rho[c], P.emr0[c]));
+            st_plane(aux + (size_t)(C + c) * P.ns, s, rho[c]);
+        }
     }
     if constexpr (C == 3) {
         V4<R> *vec = (V4<R> *)(aux + aux3_vec_off(P.ns));
@@ -591,7 +599,7 @@ __device__ __forceinline__ void dipole_core(const Phys<R> &P, R hs0, R hs1, R hs
     out[2] = P.omega_d * e2 + P.one_m_omega_d * sd2;
 }
 
-template <typename R, bool Dev>
+template <typename R, bool Dev, bool PK = false>
 __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restrict__ aux,
                                             R *__restrict__ db, uint32_t s)
 {
@@ -608,7 +616,7 @@ __device__ __forceinline__ void dipole_site(const Phys<R> &P, const R *__restric
     const V4<R> own = ld_plane(vec, s);
     R out[3];
     dipole_core<R, Dev>(
-        P, own.b, own.c, own.d, ld_plane(plane_of(aux, 5u, P.ns), s),
+        P, own.b, own.c, own.d, PK ? ld_plane((const V4<R> *)aux, s).d : ld_plane(plane_of(aux, 5u, P.ns), s),
         [&](int i) __attribute__((always_inline)) { return nb[i].a; },
         [&](int i, int a) __attribute__((always_inline)) {
             return a == 0 ? nb[i].b : (a == 1 ? nb[i].c : nb[i].d);
@@ -746,7 +754,7 @@ __device__ __forceinline__ void sc_forces_wall(const Phys<R> &P, const R *__rest
     }
 }
 
-template <typename R, bool Dev, int C, bool LE>
+template <typename R, bool Dev, int C, bool LE, bool PK = false>
 __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict__ aux,
                                           const R *__restrict__ face_aux,
                                           const R *__restrict__ db, int x, int y, int z,
@@ -775,6 +783,24 @@ __device__ __forceinline__ void sc_forces(const Phys<R> &P, const R *__restrict_
             F[1][k] = (-(P.g_br * fbr[k])) * psb;
             F[0][k] = (-(P.g_br * frb[k])) * psr;
         }
+    } else if constexpr (PK) {
+        const R dbx[3] = {ld_plane(db, s), ld_plane(plane_of(db, 1u, P.ns), s), ld_plane(plane_of(db, 2u, P.ns), s)};
+        const V4<R> *ps4 = (const V4<R> *)aux;
+        const V4<R> po = ld_plane(ps4, s);
+        const R ps_own[3] = {po.a, po.b, po.c};
+        V4<R> pn[18];
+#pragma unroll
+        for (int i = 0; i < 18; i++)
+            pn[i] = ld_plane(ps4, sidx(wrap1(x + CX[i], P.nx), wrap1(y + CY[i], P.ny), wrap1(z + CZ[i], P.nz), P.nx, P.ny));
+        sc_forces3<R, Dev>(
+            P, dbx, ps_own,
+            [&](int i, int c) __attribute__((always_inline)) {
+                return c == 0 ? pn[i].a : (c == 1 ? pn[i].b : pn[i].c);
+            },
+            [&](int i, int a) __attribute__((always_inline)) {
+                return aux_at<R, false>(db, nullptr, P, a, x + CX[i], y + CY[i], z + CZ[i]);
+            },
+            F);
     } else {
         const R dbx[3] = {ld_plane(db, s), ld_plane(plane_of(db, 1u, P.ns), s), ld_plane(plane_of(db, 2u, P.ns), s)};
         const R ps_own[3] = {ld_plane(aux, s), ld_plane(plane_of(aux, 1u, P.ns), s),
@@ -1000,7 +1026,8 @@ __device__ __forceinline__ void bgk_site(const Phys<R> &P, const R (&n)[C][19], 
 // k_collide: pull, Shan-Chen forces from the aux field, BGK; the second pass
 // of the two-pass step (k_moments then k_collide).
 // NT: bit 0 loads, bit 1 stores the populations with the non-temporal hint
-template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false, int NT = 0>
+template <typename R, bool Dev, int C, bool LE, int BD, bool Rock = false, bool ZH = false, int NT = 0,
+          bool PK = false>
 __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restrict__ f,
                                              const R *__restrict__ face_pc,
                                              const R *__restrict__ aux,
@@ -1031,7 +1058,7 @@ __device__ __forceinline__ void collide_site(const Phys<R> &P, const R *__restri
         // the force stencil first, then the pulls (a compiler barrier keeps
         // them after it): the populations are not live while the stencil's
         // loads are in flight, which leaves the registers to batch those loads
-        sc_forces<R, Dev, C, LE>(P, aux, face_aux, db, x, y, z, s, F);
+        sc_forces<R, Dev, C, LE, PK>(P, aux, face_aux, db, x, y, z, s, F);
         asm volatile("" ::: "memory");
 #pragma unroll
         for (int i = 0; i < 19; i++)
@@ -1764,7 +1791,7 @@ __device__ __forceinline__ bool batch_form_is(int bdist)
         return bdist == BD;
 }
 
-template <typename R, bool Dev, int C>
+template <typename R, bool Dev, int C, bool PK = false>
 __global__ __launch_bounds__(256) void k_batch_moments(const BatchLat<R> *__restrict__ lat,
                                                        const uint2 *__restrict__ items,
                                                        uint32_t nitems, const R *__restrict__ f,
@@ -1779,12 +1806,12 @@ __global__ __launch_bounds__(256) void k_batch_moments(const BatchLat<R> *__rest
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
         if (s < L.P.ns)
-            moments_site<R, Dev, C, false>(L.P, f + L.f_off, nullptr, C == 3 ? db + L.db_off : nullptr,
-                                           aux + L.aux_off, s);
+            moments_site<R, Dev, C, false, false, false, PK>(L.P, f + L.f_off, nullptr,
+                                                             C == 3 ? db + L.db_off : nullptr, aux + L.aux_off, s);
     }
 }
 
-template <typename R, bool Dev>
+template <typename R, bool Dev, bool PK = false>
 __global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restrict__ lat,
                                                       const uint2 *__restrict__ items,
                                                       uint32_t nitems, const R *__restrict__ aux,
@@ -1797,11 +1824,11 @@ __global__ __launch_bounds__(256) void k_batch_dipole(const BatchLat<R> *__restr
         uint2 w = batch_item(items, it);
         const BatchLat<R> &L = lat[w.x];
         uint32_t s = w.y + threadIdx.x;
-        if (s < L.P.ns) dipole_site<R, Dev>(L.P, aux + L.aux_off, db + L.db_off, s);
+        if (s < L.P.ns) dipole_site<R, Dev, PK>(L.P, aux + L.aux_off, db + L.db_off, s);
     }
 }
 
-template <typename R, bool Dev, int C, int BD, bool NT>
+template <typename R, bool Dev, int C, int BD, bool NT, bool PK = false>
 __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__restrict__ lat,
                                                        const uint2 *__restrict__ items,
                                                        uint32_t nitems, const R *__restrict__ f,
@@ -1819,7 +1846,7 @@ __global__ __launch_bounds__(256) void k_batch_collide(const BatchLat<R> *__rest
         // non-temporal stores of the populations (NT): -2% per step of the
         // 48-lattice ternary ensemble; non-temporal loads were slower
         if (s < L.P.ns && batch_form_is<BD>(L.P.bdist))
-            collide_site<R, Dev, C, false, BD, false, false, NT ? 2 : 0>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
+            collide_site<R, Dev, C, false, BD, false, false, NT ? 2 : 0, PK>(L.P, f + L.f_off, nullptr, aux + L.aux_off, nullptr,
                                                                C == 3 ? db + L.db_off : nullptr, fout + L.f_off, s);
     }
 }

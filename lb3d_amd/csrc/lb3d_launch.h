@@ -551,18 +551,26 @@ class BatchLauncherT : public BatchLauncher {
         std::memcpy(dst, &L, sizeof(L));
     }
     void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg, const void *f,
-               void *fout, void *aux, void *db, int ncomp, hipStream_t s) override
+               void *fout, void *aux, void *db, int ncomp, hipStream_t s, bool packed) override
     {
         const BatchLat<R> *L = (const BatchLat<R> *)lat;
         const uint2 *it = (const uint2 *)items;
         if (ph == 0) {
-            if (ncomp == 3)
+            if (ncomp == 3 && packed)
+                k_batch_moments<R, Dev, 3, true>
+                    <<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)db, (R *)aux, xg);
+            else if (ncomp == 3)
                 k_batch_moments<R, Dev, 3><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)db, (R *)aux, xg);
             else
                 k_batch_moments<R, Dev, 2><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, nullptr, (R *)aux, xg);
         } else if (ph == 1) {
-            if (ncomp == 3) k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db, xg);
+            if (ncomp == 3 && packed)
+                k_batch_dipole<R, Dev, true><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db, xg);
+            else if (ncomp == 3)
+                k_batch_dipole<R, Dev><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)aux, (R *)db, xg);
         } else {
+            // the ternary collide reads the packed layout only: the moments
+            // phase before it is launched with packed = true
             // every lattice's collide runs in the launch of its own equilibrium
             // form: bdist 0, bdist 2, or one launch for all the others
             for (int bd : {0, 2, BD_GENERAL}) {
@@ -615,17 +623,17 @@ class BatchLauncherT : public BatchLauncher {
                     void *fout, void *aux, void *db, int ncomp, hipStream_t s)
     {
         if (ncomp == 3 && bd == 0)
-            k_batch_collide<R, Dev, 3, 0, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                     (const R *)db, (R *)fout, xg);
+            k_batch_collide<R, Dev, 3, 0, NT, true><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                           (const R *)db, (R *)fout, xg);
         else if (ncomp == 3 && bd == BD_GENERAL)
-            k_batch_collide<R, Dev, 3, BD_GENERAL, NT><<<nwg, BLOCK, 0, s>>>(
+            k_batch_collide<R, Dev, 3, BD_GENERAL, NT, true><<<nwg, BLOCK, 0, s>>>(
                 L, it, nitems, (const R *)f, (const R *)aux, (const R *)db, (R *)fout, xg);
         else if (bd == BD_GENERAL)
             k_batch_collide<R, Dev, 2, BD_GENERAL, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f,
                                                                               (const R *)aux, nullptr, (R *)fout, xg);
         else if (ncomp == 3)
-            k_batch_collide<R, Dev, 3, 2, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
-                                                                     (const R *)db, (R *)fout, xg);
+            k_batch_collide<R, Dev, 3, 2, NT, true><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
+                                                                           (const R *)db, (R *)fout, xg);
         else if (bd == 0)
             k_batch_collide<R, Dev, 2, 0, NT><<<nwg, BLOCK, 0, s>>>(L, it, nitems, (const R *)f, (const R *)aux,
                                                                      nullptr, (R *)fout, xg);

@@ -96,9 +96,13 @@ class BatchLauncher {
     // host image of the lattice table entry k
     virtual void fill_lat(void *dst, const lb3d_params &p, uint64_t f_off, uint64_t aux_off,
                           uint64_t db_off) const = 0;
-    // phase ph (0 moments, 1 dipole, 2 collide) over nitems work items
+    // phase ph (0 moments, 1 dipole, 2 collide) over nitems work items;
+    // packed: the ternary moments and dipole phases use the packed layout of
+    // the moments field (one psi_r, psi_b, psi_s, rho_s 4-vector per site),
+    // else the planes; the ternary collide phase always reads the packed one
     virtual void phase(int ph, const void *lat, const void *items, uint32_t nitems, unsigned nwg,
-                       const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s) = 0;
+                       const void *f, void *fout, void *aux, void *db, int ncomp, hipStream_t s,
+                       bool packed) = 0;
     // the collide-moments plan (FP32 ternary only): k_batch_cm over nitems
     // tiles (uint4 items) with work-groups of threads = 256 or 512, and k_batch_edge_moments over nitems runs of 256
     // edge sites (uint2 items into the site list)
