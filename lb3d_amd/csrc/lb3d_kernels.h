@@ -31,6 +31,9 @@ constexpr int BD_GENERAL = 9;
 // psi_b, rho_r, rho_b.  Ternary: planes psi_r, psi_b, psi_s, rho_r, rho_b,
 // rho_s, then at a 4-element boundary one 4-vector per site of what the
 // dipole relaxation reads at its 18 neighbours, (rho_b - rho_r, da_xyz).
+// The batch's three-pass plan writes its ternary field packed: in place of
+// the six planes, one 4-vector (psi_r, psi_b, psi_s, rho_s) per site from the
+// start of the field (8 ns elements <= aux_elems), then the same 4-vectors.
 constexpr int NAUX2 = 4;
 template <typename R>
 struct alignas(4 * sizeof(R)) V4 {
