@@ -20,8 +20,10 @@ The binary spinodal lattice (256^3, FP32 deviation and FP64 plain), the
 ternary 128^3 and the Lees-Edwards 256x64x64 configurations are reported in
 extra keys, each with its FP32-against-FP64 error, the FP64 floor (a 2^-51
 perturbation) and an FP64 control perturbed at FP32 resolution (2^-24).  The
-CPU baseline is the oracle (the FP64 restatement of the reference loop)
-timed on a bounded sample.
+CPU baseline is the reference LB3D itself (built by oracle/ref_build.sh, run
+under mpirun) timed on a bounded sample, with the oracle (the FP64 C
+restatement of the reference loop) beside it; the oracle alone where the
+reference build is absent.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -920,7 +922,8 @@ def main():
             (0.01, 0.01, 0.0), [c for c in (10, 100, 300, 1000, 3000) if c < s3])
     cpu = None
     if not args.no_cpu_baseline:
-        print(f"[bench] CPU baseline: oracle on {args.cpu_sample_lattices} ensemble lattices x "
+        print(f"[bench] CPU baseline: the reference (where built) and the oracle on {args.cpu_sample_lattices} "
+              f"ensemble lattices x "
               f"{args.cpu_sample_steps} steps (all threads), {args.cpu_single_size}^3 x {args.cpu_single_steps} "
               "(one core)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_sample_lattices, args.cpu_sample_steps, args.cpu_single_size,
