@@ -688,9 +688,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
         const Cand top = bc;
         uint64_t total = 0;
         for (int k = 0; k < b->nlat; k++) total += (uint64_t)b->C * 19u * b->ns[k] * b->rs;
-        // measured on the 48-lattice ensemble: 192-256 MiB -1.4% per step,
-        // smaller groups lose more to their launches than they gain
-        for (uint64_t mib : {192u, 256u, 320u})
+        // measured on the 48-lattice ensemble: 192-320 MiB groups 0.3-1.9%
+        // faster per step than one group, depending on where the group
+        // boundaries fall (hence the 32 MiB steps); 160 MiB and smaller lose
+        // more to their launches than they gain, 336 MiB and larger evict
+        // what the collide re-reads
+        for (uint64_t mib : {192u, 224u, 256u, 288u, 320u})
             if ((mib << 20) < total)
                 if (int rc = time_cand({false, top.nwg, top.xg, mib << 20})) return rc;
     }
