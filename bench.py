@@ -567,7 +567,7 @@ def run_ensemble(steps, warmup, seq_steps, nlat=48):
 
     b, edges = ensemble_batch(LB3D_FP32_DEV, nlat)
     sites = sum(e ** 3 for e in edges)
-    chosen = b.autotune(3).strip().splitlines()[-1]
+    chosen = b.autotune(10).strip().splitlines()[-1]
     print(f"[bench] ensemble autotune: {chosen}", file=sys.stderr, flush=True)
     b.time_loop(warmup)
     b.synchronize()

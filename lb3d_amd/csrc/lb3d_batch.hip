@@ -697,6 +697,12 @@ int lb3d_gpu_batch_autotune(lb3d_batch *b, int32_t trial_steps, char *report, in
             if ((mib << 20) < total)
                 if (int rc = time_cand({false, top.nwg, top.xg, mib << 20})) return rc;
     }
+    if (!bc.seq && !bc.cols && bc.gb) {  // the XCD chunk again, with the chosen groups
+        const Cand top = bc;
+        for (uint32_t xg : {4u, 16u, 64u, 256u})
+            if (xg != top.xg)
+                if (int rc = time_cand({false, top.nwg, xg, top.gb})) return rc;
+    }
     if (g.db_save) {
         BTRY(hipMemcpyAsync(b->db, g.db_save, g.db_bytes, hipMemcpyDeviceToDevice, b->stream));
         BTRY(hipStreamSynchronize(b->stream));
